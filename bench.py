@@ -1,0 +1,172 @@
+"""Headline benchmark: TRTC_Sort_By_Key on 2^28 uint32 keys + uint32 payload
+on one MI355X, with the HBM-roofline fraction of the dominant kernel and the
+host baseline of the same call.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n LOG2] [--no-cpu] [--no-variants]
+
+One step = one Sort_By_Key of N = 2^28 pairs.  Every step sorts its own
+pristine copy of the input, all copies resident in HBM before the timed region
+starts (no restore inside it).  Prints exactly one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=28, help="log2 of the number of pairs")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        log("bench.py measures one GPU only: the primitive core runs on a single MI355X "
+            "(multi-GPU sharding is a design note, not a measured path); use --gpus 1")
+        sys.exit(2)
+
+    if not os.path.exists(os.path.join(ROOT, "thrustrtc_amd", "libPyThrustRTC.so")):
+        import __graft_entry__
+        __graft_entry__.build()
+
+    import numpy as np
+    import thrustrtc_amd as trtc
+    from thrustrtc_amd.native import native
+    from thrustrtc_amd import workloads as wl
+
+    N = 1 << a.n
+    K, W = a.steps, a.warmup
+
+    def make_copies(keys, vals, count):
+        src_k = trtc.device_vector_from_numpy(keys)
+        src_v = trtc.device_vector_from_numpy(vals)
+        ks, vs = [], []
+        for _ in range(count):
+            dk = trtc.device_vector("uint32_t" if keys.dtype == np.uint32 else "float", N)
+            dv = trtc.device_vector("uint32_t", N)
+            trtc.Copy(src_k, dk)
+            trtc.Copy(src_v, dv)
+            ks.append(dk)
+            vs.append(dv)
+        trtc.Wait()
+        return ks, vs
+
+    def run(keys, steps, warmup, check=False, profile=False):
+        vals = np.arange(N, dtype=np.uint32)
+        ks, vs = make_copies(keys, vals, steps + warmup)
+        for i in range(warmup):
+            trtc.Sort_By_Key(ks[i], vs[i])
+        trtc.Wait()
+        if profile:
+            native.n_profile_begin()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            trtc.Sort_By_Key(ks[i], vs[i])
+        trtc.Wait()
+        t1 = time.perf_counter()
+        prof = None
+        if profile:
+            buf = __import__("ctypes").create_string_buffer(1 << 16)
+            native.n_profile_end(buf, 1 << 16)
+            prof = json.loads(buf.value.decode())
+        ok = None
+        if check:
+            gk = ks[-1].to_host()
+            gv = vs[-1].to_host()
+            ok = bool((gk.view(np.uint32) == keys[gv].view(np.uint32)).all())
+            if keys.dtype == np.uint32:
+                ok = ok and bool((gk[1:] >= gk[:-1]).all())
+                # stable: equal keys keep increasing payload
+                eq = gk[1:] == gk[:-1]
+                ok = ok and bool((gv[1:][eq] > gv[:-1][eq]).all())
+        del ks, vs
+        return (t1 - t0) / steps, prof, ok
+
+    log(f"generating 2^{a.n} uniform uint32 keys (mt19937 seed 2)")
+    keys = wl.u32_uniform(N, 2)
+    sec, prof, ok = run(keys, K, W, check=True, profile=True)
+    if not ok:
+        log("ERROR: sorted output failed the sortedness/stability/permutation check")
+        sys.exit(1)
+    ms = sec * 1e3
+    gkeys = N / sec / 1e9
+    log(f"Sort_By_Key 2^{a.n} u32 uniform: {ms:.3f} ms/step, {gkeys:.2f} Gkeys/s, kernels: {prof}")
+
+    # dominant kernel: one onesweep digit pass (reads 8 B/pair, writes 8 B/pair)
+    pass_name = [k for k in prof if k.startswith("trtc_radix_pass_k<4, 4,")][0]
+    pass_ms = prof[pass_name]["total_ms"] / prof[pass_name]["count"]
+    pass_bytes = 16.0 * N
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
+    model_gbs = 68.0 * N / sec / 1e9
+
+    variants = {}
+    if not a.no_variants:
+        for name, gen in [("u32_low_entropy", lambda: wl.u32_low_entropy(N, 2)),
+                          ("f32_normal_signed_zeros", lambda: wl.f32_normal_signed_zeros(N, 3, 4))]:
+            kv = gen()
+            s2, _, ok2 = run(kv, max(2, K // 2), 1, check=True)
+            variants[name] = {"gkeys_per_s": round(N / s2 / 1e9, 3), "ms_per_step": round(s2 * 1e3, 4),
+                              "pct_hbm_68N": round(100 * 68.0 * N / s2 / 1e9 / HBM_PEAK_GBS, 2),
+                              "bit_exact_check": ok2}
+            log(f"variant {name}: {variants[name]}")
+            del kv
+
+    cpu = None
+    if not a.no_cpu:
+        from helpers import Oracle
+        orc = Oracle()
+        ck = keys.copy()
+        cv = np.arange(N, dtype=np.uint32)
+        t0 = time.perf_counter()
+        orc.L.bl_sort_pairs_u32(ck.ctypes.data, cv.ctypes.data, N)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(N / dt / 1e9, 4), "unit": "Gkeys/s", "cores": int(orc.L.bl_threads()),
+               "kind": "port",
+               "sample": f"the full workload: one __gnu_parallel::stable_sort of 2^{a.n} (key, index) pairs "
+                         f"(same keys), {dt:.2f} s"}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "Gkeys/s Sort_By_Key 2^28 u32 + %HBM roofline; served-mix p99 vs per-call",
+        "value": round(gkeys, 4),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"TRTC_Sort_By_Key, 2^{a.n} uint32 keys uniform over [0,2^32) (mt19937 seed 2) "
+                               f"+ uint32 original-index payload, ascending (Less), stable",
+                   "n": N, "key_bits": 32, "value_bits": 32, "digit_bits": 8, "passes": 4},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": pass_name, "kernel_avg_ms": round(pass_ms, 4),
+                     "bytes_per_launch": pass_bytes},
+        "model_68N": {"gbs": round(model_gbs, 1), "pct_hbm": round(100 * model_gbs / HBM_PEAK_GBS, 2)},
+        "kernels": prof,
+        "variants": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

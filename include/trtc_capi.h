@@ -1,0 +1,149 @@
+/* C ABI of the ThrustRTC primitive core on MI355X (libPyThrustRTC.so).
+ *
+ * Every entry point keeps the signature, argument meaning, ownership rule and
+ * error convention of the reference's FFI header python/api.h (cited per
+ * line), so the reference's Python (cffi/ctypes), JNI and C++/CLI bindings
+ * bind it unchanged.  Plain pointers and sizes only.
+ *
+ * Handles are heap objects created by n_*_create and freed by the matching
+ * n_*_destroy / n_dv_destroy.  Return conventions: int 0 = ok, -1 = internal
+ * error, -2 = wrong number of arguments, -100 = cannot initialise; unsigned
+ * 0xFFFFFFFF = error; NULL pointer = error.  A NULL functor or init argument
+ * selects the reference's default overload (Less / Plus / EqualTo / zero).
+ */
+#ifndef TRTC_CAPI_H
+#define TRTC_CAPI_H
+
+#define TRTC_CAPI __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- utils (api.h:12-17) ---- */
+TRTC_CAPI void* n_string_array_create(unsigned long long size, const char* const* strs); /* api.h:12 */
+TRTC_CAPI void n_string_array_destroy(void* ptr_arr);                                    /* api.h:13 */
+TRTC_CAPI void* n_pointer_array_create(unsigned long long size, const void* const* ptrs); /* api.h:14 */
+TRTC_CAPI void n_pointer_array_destroy(void* ptr_arr);                                    /* api.h:15 */
+TRTC_CAPI void* n_dim3_create(unsigned x, unsigned y, unsigned z);                        /* api.h:16 */
+TRTC_CAPI void n_dim3_destroy(void* cptr);                                                /* api.h:17 */
+
+/* ---- context (api.h:20-43) ---- */
+TRTC_CAPI void n_set_libnvrtc_path(const char* path);   /* api.h:20; sets the hipRTC library path */
+TRTC_CAPI int n_trtc_try_init(void);                    /* api.h:21; 0 or -100 */
+TRTC_CAPI int n_get_ptx_arch(void);                     /* api.h:22; gfx number, e.g. 950 */
+TRTC_CAPI void n_set_verbose(unsigned verbose);         /* api.h:23 */
+TRTC_CAPI void n_set_kernel_debug(unsigned kernel_debug); /* api.h:24 */
+TRTC_CAPI void n_add_include_dir(const char* dir);      /* api.h:25 */
+TRTC_CAPI void n_add_built_in_header(const char* filename, const char* filecontent); /* api.h:26 */
+TRTC_CAPI void n_add_inlcude_filename(const char* fn);  /* api.h:27 (sic) */
+TRTC_CAPI void n_add_code_block(const char* line);      /* api.h:28 */
+TRTC_CAPI void n_add_constant_object(const char* name, void* cptr); /* api.h:29 */
+TRTC_CAPI void n_wait(void);                            /* api.h:30 */
+
+TRTC_CAPI void* n_kernel_create(void* ptr_param_list, const char* body);  /* api.h:32 */
+TRTC_CAPI void n_kernel_destroy(void* cptr);                              /* api.h:33 */
+TRTC_CAPI int n_kernel_num_params(void* cptr);                            /* api.h:34 */
+TRTC_CAPI int n_kernel_calc_optimal_block_size(void* ptr_kernel, void* ptr_arg_list, unsigned sharedMemBytes); /* api.h:35 */
+TRTC_CAPI int n_kernel_calc_number_blocks(void* ptr_kernel, void* ptr_arg_list, int sizeBlock, unsigned sharedMemBytes); /* api.h:36 */
+TRTC_CAPI int n_kernel_launch(void* ptr_kernel, void* ptr_gridDim, void* ptr_blockDim, void* ptr_arg_list, int sharedMemBytes); /* api.h:37 */
+
+TRTC_CAPI void* n_for_create(void* ptr_param_list, const char* name_iter, const char* body); /* api.h:39 */
+TRTC_CAPI void n_for_destroy(void* cptr);                                 /* api.h:40 */
+TRTC_CAPI int n_for_num_params(void* cptr);                               /* api.h:41 */
+TRTC_CAPI int n_for_launch(void* cptr, int begin, int end, void* ptr_arg_list); /* api.h:42 */
+TRTC_CAPI int n_for_launch_n(void* cptr, int n, void* ptr_arg_list);      /* api.h:43 */
+
+/* ---- device-viewable scalars (api.h:46-75) ---- */
+TRTC_CAPI const char* n_dv_name_view_cls(void* cptr);   /* api.h:46 */
+TRTC_CAPI void n_dv_destroy(void* cptr);                /* api.h:47 */
+TRTC_CAPI void* n_dvint8_create(int v);                 /* api.h:48 */
+TRTC_CAPI int n_dvint8_value(void* cptr);               /* api.h:49 */
+TRTC_CAPI void* n_dvuint8_create(unsigned v);           /* api.h:50 */
+TRTC_CAPI unsigned n_dvuint8_value(void* cptr);         /* api.h:51 */
+TRTC_CAPI void* n_dvint16_create(int v);                /* api.h:52 */
+TRTC_CAPI int n_dvint16_value(void* cptr);              /* api.h:53 */
+TRTC_CAPI void* n_dvuint16_create(unsigned v);          /* api.h:54 */
+TRTC_CAPI unsigned n_dvuint16_value(void* cptr);        /* api.h:55 */
+TRTC_CAPI void* n_dvint32_create(int v);                /* api.h:56 */
+TRTC_CAPI int n_dvint32_value(void* cptr);              /* api.h:57 */
+TRTC_CAPI void* n_dvuint32_create(unsigned v);          /* api.h:58 */
+TRTC_CAPI unsigned n_dvuint32_value(void* cptr);        /* api.h:59 */
+TRTC_CAPI void* n_dvint64_create(long long v);          /* api.h:60 */
+TRTC_CAPI long long n_dvint64_value(void* cptr);        /* api.h:61 */
+TRTC_CAPI void* n_dvuint64_create(unsigned long long v); /* api.h:62 */
+TRTC_CAPI unsigned long long n_dvuint64_value(void* cptr); /* api.h:63 */
+TRTC_CAPI void* n_dvfloat_create(float v);              /* api.h:64 */
+TRTC_CAPI float n_dvfloat_value(void* cptr);            /* api.h:65 */
+TRTC_CAPI void* n_dvdouble_create(double v);            /* api.h:66 */
+TRTC_CAPI double n_dvdouble_value(void* cptr);          /* api.h:67 */
+TRTC_CAPI void* n_dvbool_create(int v);                 /* api.h:68 */
+TRTC_CAPI int n_dvbool_value(void* cptr);               /* api.h:69 */
+TRTC_CAPI void* n_dvcomplex64_create(float real, float imag);   /* api.h:70 */
+TRTC_CAPI float n_dvcomplex64_real(void* cptr);                 /* api.h:71 */
+TRTC_CAPI float n_dvcomplex64_imag(void* cptr);                 /* api.h:72 */
+TRTC_CAPI void* n_dvcomplex128_create(double real, double imag); /* api.h:73 */
+TRTC_CAPI double n_dvcomplex128_real(void* cptr);               /* api.h:74 */
+TRTC_CAPI double n_dvcomplex128_imag(void* cptr);               /* api.h:75 */
+
+/* ---- vectors (api.h:78-84) ---- */
+TRTC_CAPI const char* n_dvvectorlike_name_elem_cls(void* cptr);  /* api.h:78 */
+TRTC_CAPI unsigned long long n_dvvectorlike_size(void* cptr);    /* api.h:79 */
+TRTC_CAPI void* n_dvvector_create(const char* elem_cls, unsigned long long size, void* hdata); /* api.h:80 */
+TRTC_CAPI void n_dvvector_to_host(void* cptr, void* hdata, unsigned long long begin, unsigned long long end); /* api.h:81 */
+TRTC_CAPI void* n_dvvector_from_dvs(void* ptr_dvs);              /* api.h:82 */
+TRTC_CAPI void* n_dvvectoradaptor_create(const char* elem_cls, unsigned long long size, void* data); /* api.h:83 */
+TRTC_CAPI void* n_dvrange_create(void* ptr_in, unsigned long long begin, unsigned long long end); /* api.h:84 */
+
+/* ---- functors (api.h:100-101) ---- */
+TRTC_CAPI void* n_functor_create(void* ptr_dvs, void* ptr_names, void* ptr_functor_params, const char* code_body); /* api.h:100 */
+TRTC_CAPI void* n_built_in_functor_create(const char* name_built_in_view_cls); /* api.h:101 */
+
+/* ---- algorithms on the hot path ---- */
+TRTC_CAPI int n_fill(void* ptr_vec, void* ptr_value);                 /* api.h:104 */
+TRTC_CAPI int n_copy(void* ptr_in, void* ptr_out);                    /* api.h:124 */
+TRTC_CAPI void* n_reduce(void* ptr_vec, void* ptr_init, void* ptr_bin_op); /* api.h:130; new DV, caller frees */
+TRTC_CAPI int n_is_sorted(void* ptr_vec, void* ptr_comp);             /* api.h:142 */
+TRTC_CAPI int n_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_binary_op); /* api.h:145 */
+TRTC_CAPI int n_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_init, void* ptr_binary_op); /* api.h:146 */
+TRTC_CAPI int n_inclusive_scan_by_key(void* ptr_vec_key, void* ptr_vec_value, void* ptr_vec_out, void* ptr_binary_pred, void* ptr_binary_op); /* api.h:147 */
+TRTC_CAPI int n_exclusive_scan_by_key(void* ptr_vec_key, void* ptr_vec_value, void* ptr_vec_out, void* ptr_init, void* ptr_binary_pred, void* ptr_binary_op); /* api.h:148 */
+TRTC_CAPI unsigned n_copy_if(void* ptr_in, void* ptr_out, void* ptr_pred); /* api.h:153 */
+TRTC_CAPI unsigned n_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_pred); /* api.h:154 */
+TRTC_CAPI int n_sort(void* ptr_vec, void* ptr_comp);                  /* api.h:189 */
+TRTC_CAPI int n_sort_by_key(void* ptr_keys, void* ptr_values, void* ptr_comp); /* api.h:190 */
+
+/* ---- extensions: no counterpart in the reference ----
+ * Batched segmented primitives and the request server.  Callers: the batched
+ * benchmark and services that submit many independent sorts / scans. */
+/* Sort each segment [offsets[s], offsets[s+1]) of keys/values independently
+ * (offsets: host array, num_segments + 1 entries); comp NULL = Less. */
+TRTC_CAPI int n_segmented_sort_by_key(void* ptr_keys, void* ptr_values, const unsigned long long* offsets,
+                                      unsigned long long num_segments, void* ptr_comp);
+/* Inclusive Plus scan of each segment independently. */
+TRTC_CAPI int n_segmented_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
+                                         unsigned long long num_segments);
+/* Stream used by the calling thread (a hipStream_t; NULL = null stream). */
+TRTC_CAPI void n_set_stream(void* stream);
+TRTC_CAPI void* n_get_stream(void);
+/* Device pointer of a DVVector / DVVectorAdaptor (NULL for other kinds). */
+TRTC_CAPI void* n_dvvectorlike_data(void* cptr);
+/* Compile, without touching a device, the kernel template instantiation
+ * `name_expr` (e.g. "trtc_copy_if_k<VectorView<int32_t>, ..., _S_...>") with
+ * the definitions of every functor struct it names, for target `arch`
+ * (e.g. "gfx950").  0 = compiled, -1 = compile error (log on stderr).
+ * Callers: offline cache warm-up and build-time checks of functor code. */
+TRTC_CAPI int n_jit_compile_check(const char* arch, const char* name_expr);
+/* Kernel timing: between n_profile_begin and n_profile_end every kernel
+ * launch is bracketed by HIP events on its stream; n_profile_end synchronises
+ * and writes {"<kernel>": {"count": c, "total_ms": t}, ...} into json_out
+ * (capacity bytes, NUL-terminated) and returns the full JSON length.
+ * Callers: the benchmark's per-kernel roofline figures. */
+TRTC_CAPI int n_profile_begin(void);
+TRTC_CAPI unsigned long long n_profile_end(char* json_out, unsigned long long capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

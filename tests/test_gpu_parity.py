@@ -1,0 +1,389 @@
+"""GPU parity: the HIP path (through libPyThrustRTC.so's C ABI, driven by the
+reference-shaped Python API) against the oracle on the same seeded inputs.
+
+Integer / index / byte results are compared bit-exactly; float reductions and
+scans against the FP64 oracle within the tolerance stated in each test.
+"""
+import json
+import os
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+from helpers import Oracle, workloads, NP
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
+
+
+@pytest.fixture(scope="module")
+def trtc():
+    import thrustrtc_amd
+    return thrustrtc_amd
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return workloads()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------- known answers
+def test_known_answers(trtc):
+    for c in GOLDEN["sort"]:
+        d = trtc.device_vector_from_list(c["in"], c["elem"])
+        trtc.Sort(d, trtc.Greater() if c["comp"] == "Greater" else trtc.Less())
+        assert d.to_host().tolist() == c["out"]
+    for c in GOLDEN["sort_by_key"]:
+        k = trtc.device_vector_from_list(c["keys"], c["kelem"])
+        v = trtc.device_vector_from_list(c["values"], c["velem"])
+        trtc.Sort_By_Key(k, v, trtc.Greater() if c["comp"] == "Greater" else None)
+        assert k.to_host().tolist() == c["keys_out"] and v.to_host().tolist() == c["values_out"]
+    ops = {"Plus": trtc.Plus, "Maximum": trtc.Maximum}
+    for c in GOLDEN["inclusive_scan"]:
+        d = trtc.device_vector_from_list(c["in"], c["elem"])
+        trtc.Inclusive_Scan(d, d, ops[c["op"]]())
+        assert d.to_host().tolist() == c["out"]
+    for c in GOLDEN["exclusive_scan"]:
+        d = trtc.device_vector_from_list(c["in"], c["elem"])
+        if c["init"] == 0 and c["op"] == "Plus":
+            trtc.Exclusive_Scan(d, d)
+        else:
+            trtc.Exclusive_Scan(d, d, trtc.DVInt32(c["init"]), ops[c["op"]]())
+        assert d.to_host().tolist() == c["out"]
+    for c in GOLDEN["scan_by_key"]:
+        k = trtc.device_vector_from_list(c["keys"], "int32_t")
+        v = trtc.device_vector_from_list(c["values"], "int32_t")
+        if not c["exclusive"]:
+            trtc.Inclusive_Scan_By_Key(k, v, v)
+        elif c["init"] == 0:
+            trtc.Exclusive_Scan_By_Key(k, v, v)
+        else:
+            trtc.Exclusive_Scan_By_Key(k, v, v, trtc.DVInt32(c["init"]))
+        assert v.to_host().tolist() == c["out"]
+    for c in GOLDEN["reduce"]:
+        d = trtc.device_vector_from_list(c["in"], c["elem"])
+        if c["init"] is None:
+            r = trtc.Reduce(d)
+        elif c["op"] == "Plus":
+            r = trtc.Reduce(d, trtc.DVInt32(c["init"]))
+        else:
+            r = trtc.Reduce(d, trtc.DVInt32(c["init"]), ops[c["op"]]())
+        assert r == c["out"]
+    is_even = trtc.Functor({}, ["x"], "         return x % 2 == 0;\n")
+    for c in GOLDEN["copy_if"]:
+        d = trtc.device_vector_from_list(c["in"], "int32_t")
+        o = trtc.device_vector("int32_t", len(c["in"]))
+        cnt = trtc.Copy_If(d, o, is_even)
+        assert o.to_host(0, cnt).tolist() == c["out"]
+    for c in GOLDEN["copy_if_stencil"]:
+        d = trtc.device_vector_from_list(c["in"], "int32_t")
+        s = trtc.device_vector_from_list(c["stencil"], "int32_t")
+        o = trtc.device_vector("int32_t", len(c["in"]))
+        cnt = trtc.Copy_If_Stencil(d, s, o, is_even)
+        assert o.to_host(0, cnt).tolist() == c["out"]
+
+
+# ---------------------------------------------------------------- sort
+SORT_SIZES = [1, 2, 3, 255, 4095, 4096, 4097, 65537, 1 << 20]
+
+
+def _keys(wl, kind, n, seed):
+    if kind == "u32":
+        return wl.u32_uniform(n, seed), "uint32_t"
+    if kind == "u32_low":
+        return wl.u32_low_entropy(n, seed), "uint32_t"
+    if kind == "f32":
+        return wl.f32_normal_signed_zeros(n, seed, seed + 1), "float"
+    if kind == "i32":
+        return wl.i32_range(n, seed, -2**31, 2**31 - 1), "int32_t"
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["u32", "u32_low", "f32", "i32"])
+@pytest.mark.parametrize("n", SORT_SIZES)
+@pytest.mark.parametrize("greater", [False, True])
+def test_sort_by_key_parity(trtc, orc, wl, kind, n, greater):
+    say("sort_by_key", kind, n, greater)
+    keys, elem = _keys(wl, kind, n, 2)
+    idx = np.arange(n, dtype=np.uint32)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(idx)
+    trtc.Sort_By_Key(dk, dv, trtc.Greater() if greater else trtc.Less())
+    perm = orc.sort_perm(keys, elem, greater)
+    gk, gv = dk.to_host(), dv.to_host()
+    # keys bit-exact (-0.0 stays -0.0), payload = the oracle's stable permutation
+    assert (gk.view(np.uint32) == keys[perm].view(np.uint32)).all()
+    assert (gv == perm.astype(np.uint32)).all()
+
+
+@pytest.mark.parametrize("elem", ["int8_t", "uint8_t", "int16_t", "uint16_t", "int64_t", "uint64_t", "double",
+                                  "float", "int32_t", "uint32_t"])
+def test_sort_types(trtc, orc, wl, elem):
+    say("sort types", elem)
+    n = 70001
+    raw = wl.u32_uniform(2 * n, 5).view(np.uint64) if NP[elem]().itemsize == 8 else wl.u32_uniform(n, 5)
+    keys = raw.view(np.uint8)[: n * NP[elem]().itemsize].view(NP[elem]).copy()
+    if elem in ("float", "double"):
+        keys = np.where(np.isnan(keys), 0, keys).astype(NP[elem])
+        keys[::97] = -0.0
+    for greater in (False, True):
+        d = trtc.device_vector_from_numpy(keys)
+        trtc.Sort(d, trtc.Greater() if greater else None)
+        want = keys[orc.sort_perm(keys, elem, greater)]
+        assert (d.to_host().view(np.uint8) == want.view(np.uint8)).all(), (elem, greater)
+
+
+@pytest.mark.parametrize("velem", ["int8_t", "int16_t", "int64_t", "double"])
+def test_sort_by_key_value_widths(trtc, orc, wl, velem):
+    n = 30000
+    keys = wl.u32_low_entropy(n, 8)
+    vals = (np.arange(n) * 7 + 3).astype(NP[velem])
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(vals)
+    trtc.Sort_By_Key(dk, dv)
+    perm = orc.sort_perm(keys, "uint32_t")
+    assert (dk.to_host() == keys[perm]).all()
+    assert (dv.to_host() == vals[perm]).all()
+
+
+def test_sort_values_longer_than_keys(trtc, orc, wl):
+    # only the first keys.size() values are permuted (radix_sort.cpp:328-370)
+    keys = wl.u32_uniform(1000, 3)
+    vals = np.arange(1500, dtype=np.int32)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(vals)
+    trtc.Sort_By_Key(dk, dv)
+    perm = orc.sort_perm(keys, "uint32_t")
+    got = dv.to_host()
+    assert (got[:1000] == perm).all() and (got[1000:] == vals[1000:]).all()
+
+
+def test_sort_all_equal_and_sorted(trtc):
+    for arr in [np.full(100000, 7, np.uint32), np.arange(100000, dtype=np.uint32),
+                np.arange(100000, 0, -1).astype(np.uint32)]:
+        dk = trtc.device_vector_from_numpy(arr)
+        dv = trtc.device_vector_from_numpy(np.arange(len(arr), dtype=np.uint32))
+        trtc.Sort_By_Key(dk, dv)
+        perm = np.argsort(arr, kind="stable")
+        assert (dk.to_host() == arr[perm]).all() and (dv.to_host() == perm).all()
+
+
+# ---------------------------------------------------------------- scan
+SCAN_SIZES = [1, 2, 1000, 4095, 4096, 4097, 100003, 1 << 20]
+
+
+@pytest.mark.parametrize("n", SCAN_SIZES)
+def test_scan_i32_parity(trtc, orc, wl, n):
+    say("scan i32", n)
+    x = wl.i32_range(n, 0, -100, 100)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector("int32_t", n)
+    trtc.Inclusive_Scan(d, o)
+    assert (o.to_host() == orc.inclusive_scan(x, "int32_t")).all()
+    trtc.Exclusive_Scan(d, o, trtc.DVInt32(17))
+    assert (o.to_host() == orc.exclusive_scan(x, "int32_t", 17)).all()
+    trtc.Inclusive_Scan(d, o, trtc.Maximum())
+    assert (o.to_host() == orc.inclusive_scan(x, "int32_t", "Maximum")).all()
+    # in place
+    trtc.Exclusive_Scan(d, d)
+    assert (d.to_host() == orc.exclusive_scan(x, "int32_t", 0)).all()
+
+
+@pytest.mark.parametrize("elem", ["int64_t", "uint32_t", "double", "float", "int8_t", "uint16_t"])
+def test_scan_types(trtc, orc, wl, elem):
+    n = 50001
+    x = wl.i32_range(n, 4, -50, 50).astype(NP[elem])
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector(elem, n)
+    trtc.Inclusive_Scan(d, o)
+    want = orc.inclusive_scan(x, elem)
+    if elem in ("float", "double"):
+        # integer-valued, |partial sums| < 2^24: exact in any association
+        assert (o.to_host() == want).all()
+    else:
+        assert (o.to_host() == want).all()
+    trtc.Inclusive_Scan(d, o, trtc.Minimum())
+    assert (o.to_host() == orc.inclusive_scan(x, elem, "Minimum")).all()
+
+
+def test_scan_float_tolerance(trtc, wl):
+    n = 1 << 20
+    x = wl.f32_unit(n, 6)
+    d = trtc.device_vector_from_numpy(x)
+    trtc.Inclusive_Scan(d, d)
+    ref = np.cumsum(x.astype(np.float64))
+    got = d.to_host().astype(np.float64)
+    # relative error bound for float partial sums of positive terms
+    assert (np.abs(got - ref) <= 1e-5 * ref + 1e-6).all()
+
+
+def test_scan_user_functor_and_type_mix(trtc, orc, wl):
+    n = 20000
+    x = wl.i32_range(n, 1, -100, 100).astype(np.int8)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector("int32_t", n)
+    # int8 input scanned in int32 (the output type), op from a code string
+    op = trtc.Functor({}, ["a", "b"], "        return a + b;\n")
+    trtc.Inclusive_Scan(d, o, op)
+    assert (o.to_host() == np.cumsum(x.astype(np.int64)).astype(np.int32)).all()
+
+
+def test_scan_non_commutative_op(trtc):
+    # Pair-free non-commutative associative op: keep the first non-zero
+    n = 10000
+    x = np.zeros(n, dtype=np.int32)
+    x[[5000, 7000]] = [3, 9]
+    d = trtc.device_vector_from_numpy(x)
+    op = trtc.Functor({}, ["a", "b"], "        return a != 0 ? a : b;\n")
+    trtc.Inclusive_Scan(d, d, op)
+    want = np.zeros(n, dtype=np.int32)
+    want[5000:] = 3
+    assert (d.to_host() == want).all()
+
+
+# ---------------------------------------------------------------- reduce
+@pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 100003, 1 << 22])
+def test_reduce_parity(trtc, orc, wl, n):
+    say("reduce", n)
+    x = wl.i32_range(n, 3, -1000, 1000)
+    d = trtc.device_vector_from_numpy(x)
+    assert trtc.Reduce(d) == orc.reduce(x, "int32_t")
+    assert trtc.Reduce(d, trtc.DVInt32(5)) == orc.reduce(x, "int32_t", 5)
+    assert trtc.Reduce(d, trtc.DVInt32(-5000), trtc.Maximum()) == orc.reduce(x, "int32_t", -5000, "Maximum")
+    f = wl.f32_unit(n, 6)
+    df = trtc.device_vector_from_numpy(f)
+    r = trtc.Reduce(df)
+    ref = orc.sum_f32_fp64(f)
+    assert abs(r - ref) <= 1e-5 * abs(ref) + 1e-6  # stated tolerance: rel 1e-5 vs FP64
+
+
+def test_reduce_deterministic_and_user_op(trtc, wl):
+    f = wl.f32_normal_signed_zeros(1 << 20, 9, 10)
+    d = trtc.device_vector_from_numpy(f)
+    r1, r2 = trtc.Reduce(d), trtc.Reduce(d)
+    assert np.float32(r1).view(np.uint32) == np.float32(r2).view(np.uint32)
+    op = trtc.Functor({}, ["a", "b"], "        return a > b ? a : b;\n")
+    assert trtc.Reduce(d, trtc.DVFloat(-1e30), op) == f.max()
+
+
+# ---------------------------------------------------------------- copy_if
+@pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 100003, 1 << 21])
+def test_copy_if_parity(trtc, orc, wl, n):
+    say("copy_if", n)
+    x = wl.i32_range(n, 5, -1000, 1000)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector_from_numpy(np.full(n, -7777, dtype=np.int32))
+    pred = trtc.Functor({}, ["x"], "        return x % 3 == 0;\n")
+    cnt = trtc.Copy_If(d, o, pred)
+    want = orc.copy_if_mod(x, 3, 0)
+    assert cnt == len(want)
+    got = o.to_host()
+    assert (got[:cnt] == want).all()
+    assert (got[cnt:] == -7777).all()  # untouched past the count
+
+
+def test_copy_if_captured_vector(trtc, wl):
+    # a predicate capturing a device vector and a scalar (functor with members)
+    n = 50000
+    x = wl.i32_range(n, 8, 0, 99)
+    lut = np.zeros(100, dtype=np.int32)
+    lut[[3, 50, 77]] = 1
+    d = trtc.device_vector_from_numpy(x)
+    dl = trtc.device_vector_from_numpy(lut)
+    o = trtc.device_vector("int32_t", n)
+    pred = trtc.Functor({"lut": dl, "bias": trtc.DVInt32(0)}, ["x"], "        return lut[x] + bias != 0;\n")
+    cnt = trtc.Copy_If(d, o, pred)
+    want = x[lut[x] != 0]
+    assert cnt == len(want) and (o.to_host(0, cnt) == want).all()
+
+
+# ---------------------------------------------------------------- scan_by_key
+@pytest.mark.parametrize("n", [1, 5, 4096, 4097, 100003, 1 << 20])
+def test_scan_by_key_parity(trtc, orc, wl, n):
+    say("scan_by_key", n)
+    sizes, _ = wl.segment_sizes(n, 11, 1, 5000)
+    keys = np.repeat(np.arange(len(sizes), dtype=np.int32), sizes.astype(np.int64))[:n]
+    vals = wl.i32_range(n, 12, -1000, 1000)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(vals)
+    o = trtc.device_vector("int32_t", n)
+    trtc.Inclusive_Scan_By_Key(dk, dv, o)
+    assert (o.to_host() == orc.scan_by_key(keys, vals, "int32_t", "int32_t")).all()
+    trtc.Exclusive_Scan_By_Key(dk, dv, o, trtc.DVInt32(3))
+    assert (o.to_host() == orc.scan_by_key(keys, vals, "int32_t", "int32_t", True, 3)).all()
+    trtc.Exclusive_Scan_By_Key(dk, dv, dv)  # in place
+    assert (dv.to_host() == orc.scan_by_key(keys, vals, "int32_t", "int32_t", True, 0)).all()
+
+
+def test_scan_by_key_float_keys_and_max(trtc, orc):
+    keys = np.array([0.0, -0.0, 1.0, 1.0, 2.0, -0.0] * 3000, dtype=np.float32)
+    vals = np.arange(len(keys), dtype=np.int32) % 13
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(vals)
+    o = trtc.device_vector("int32_t", len(keys))
+    trtc.Inclusive_Scan_By_Key(dk, dv, o, trtc.EqualTo(), trtc.Maximum())
+    assert (o.to_host() == orc.scan_by_key(keys, vals, "float", "int32_t", op="Maximum")).all()
+
+
+# ---------------------------------------------------------------- user kernels / context
+def test_kernel_and_for(trtc):
+    harr = np.array([1.0, 2.0, 3.0, 4.0, 5.0], dtype=np.float32)
+    darr = trtc.device_vector_from_numpy(harr)
+    kernel = trtc.Kernel(["arr_in", "arr_out", "k"], """
+    size_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= arr_in.size()) return;
+    arr_out[idx] = arr_in[idx]*k;
+    """)
+    out = trtc.device_vector("float", 5)
+    kernel.launch(1, 128, [darr, out, trtc.DVFloat(10.0)])
+    assert out.to_host().tolist() == [10, 20, 30, 40, 50]
+    darr2 = trtc.device_vector_from_list([6, 7, 8, 9, 10], "int32_t")
+    out2 = trtc.device_vector("int32_t", 5)
+    kernel.launch(1, 128, [darr2, out2, trtc.DVInt32(5)])
+    assert out2.to_host().tolist() == [30, 35, 40, 45, 50]
+    loop = trtc.For(["arr_in", "arr_out", "k"], "idx", "    arr_out[idx] = arr_in[idx]*k;\n")
+    loop.launch_n(5, [darr, out, trtc.DVFloat(2.0)])
+    assert out.to_host().tolist() == [2, 4, 6, 8, 10]
+    assert trtc.Get_PTX_Arch() == 950
+
+
+def test_multithread_for(trtc):
+    # python/test/test_multithread.py: one For launched from three threads
+    darr = trtc.device_vector_from_numpy(np.array([1.0, 2.0, 3.0, 4.0, 5.0], dtype=np.float32))
+    loop = trtc.For(["arr_in", "arr_out", "k"], "idx", "    arr_out[idx] = arr_in[idx]*k;\n")
+    results, errors = [], []
+
+    def work():
+        try:
+            o = trtc.device_vector("float", 5)
+            loop.launch_n(5, [darr, o, trtc.DVFloat(10.0)])
+            results.append(o.to_host().tolist())
+        except Exception as e:  # pragma: no cover
+            errors.append(e)
+
+    ts = [threading.Thread(target=work) for _ in range(3)]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    assert not errors and results == [[10, 20, 30, 40, 50]] * 3
+
+
+def test_bad_functor_is_an_error_not_a_crash(trtc):
+    d = trtc.device_vector_from_list([1, 2, 3], "int32_t")
+    o = trtc.device_vector("int32_t", 3)
+    bad = trtc.Functor({}, ["x"], "        return x %% ;\n")
+    with pytest.raises(SystemError):
+        trtc.Copy_If(d, o, bad)
+    good = trtc.Functor({}, ["x"], "        return x > 1;\n")
+    assert trtc.Copy_If(d, o, good) == 2

@@ -1,0 +1,28 @@
+// Onesweep radix sort kernels for 2-byte keys (all value widths, both
+// status widths).
+#include "aot_common.h"
+
+namespace {
+template <int VB, class S> void reg_pass(const char* sn)
+{
+    std::string name = std::string("trtc_radix_pass_k<2, ") + std::to_string(VB) + ", " + sn + ">";
+    TRTC_REG(name.c_str(), (&trtc_radix_pass_k<2, VB, S>));
+}
+struct RegRadix
+{
+    RegRadix()
+    {
+        TRTC_REG("trtc_radix_hist_k<2>", (&trtc_radix_hist_k<2>));
+        reg_pass<0, unsigned int>("unsigned int");
+        reg_pass<1, unsigned int>("unsigned int");
+        reg_pass<2, unsigned int>("unsigned int");
+        reg_pass<4, unsigned int>("unsigned int");
+        reg_pass<8, unsigned int>("unsigned int");
+        reg_pass<0, unsigned long long>("unsigned long long");
+        reg_pass<1, unsigned long long>("unsigned long long");
+        reg_pass<2, unsigned long long>("unsigned long long");
+        reg_pass<4, unsigned long long>("unsigned long long");
+        reg_pass<8, unsigned long long>("unsigned long long");
+    }
+} reg_radix;
+}  // namespace

@@ -1,0 +1,86 @@
+// __global__ entry points of every primitive-core kernel.
+//
+// The same templates are instantiated two ways: ahead of time by hipcc for the
+// built-in element types and functors (thrustrtc_amd/csrc/aot/*.hip), and at
+// run time by hipRTC for user functors, fancy views and other type mixes, where
+// the host asks for the instantiation by its name expression, e.g.
+//   trtc_scan_k<VectorView<int32_t>, VectorView<int32_t>, Plus, int32_t>
+// Both paths look kernels up by that same string.
+#pragma once
+#include "trtc_scan.hpp"
+#include "trtc_reduce.hpp"
+#include "trtc_copy_if.hpp"
+#include "trtc_scan_by_key.hpp"
+#include "trtc_radix.hpp"
+
+#define TRTC_BLOCK 256
+
+// Non-template utility kernels: defined in exactly one translation unit.
+#if defined(TRTC_DEFINE_UTIL_KERNELS)
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_fill_u32(unsigned* p, size_t n, unsigned v)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Byte-wise gather for value types whose size is not 1/2/4/8:
+//   dst[i] = src[idx[i]], elem_size bytes each.
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK)
+trtc_gather_bytes(const unsigned char* src, unsigned char* dst, const unsigned* idx, size_t n, size_t elem_size)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned char* s = src + (size_t)idx[i] * elem_size;
+        unsigned char* d = dst + i * elem_size;
+        for (size_t b = 0; b < elem_size; b++) d[b] = s[b];
+    }
+}
+
+// dst[i] = i (the payload of a keys-with-index sort)
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_iota_u32(unsigned* dst, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = (unsigned)i;
+}
+#endif
+
+template <class Src, class Out, class Op, class T>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_scan_k(Src src, Out out, Op op, T init, trtc::ScanArgs a)
+{
+    trtc::scan_tile<TRTC_BLOCK, trtc::ScanItems<T>::value, T>(src, out, op, init, a);
+}
+
+template <class Src, class Op, class T>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_reduce_k(Src src, Op op, T init, trtc::ReduceArgs a)
+{
+    trtc::reduce_body<TRTC_BLOCK, trtc::ReduceItems<T>::value, T>(src, op, init, a);
+}
+
+template <class In, class Sten, class Out, class Pred>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_copy_if_k(In in, Sten sten, Out out, Pred pred, trtc::CopyIfArgs a)
+{
+    typedef typename In::value_t T;
+    trtc::copy_if_tile<TRTC_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, out, pred, a);
+}
+
+template <class Keys, class Vals, class Out, class Pred, class Op, class V>
+__global__ void __launch_bounds__(TRTC_BLOCK)
+trtc_scan_by_key_k(Keys keys, Vals vals, Out out, Pred pred, Op op, V init, trtc::ScanByKeyArgs a)
+{
+    typedef typename Keys::value_t K;
+    trtc::scan_by_key_tile<TRTC_BLOCK, trtc::ScanByKeyItems<V>::value, V, K>(keys, vals, out, pred, op, init, a);
+}
+
+template <int KB>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_hist_k(trtc::RadixHistArgs a)
+{
+    trtc::radix_hist<TRTC_BLOCK, KB>(a);
+}
+
+#define TRTC_RADIX_ITEMS 16
+
+template <int KB, int VB, class S>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a)
+{
+    trtc::radix_pass<TRTC_BLOCK, TRTC_RADIX_ITEMS, KB, VB, S>(a);
+}

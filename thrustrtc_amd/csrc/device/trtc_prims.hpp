@@ -1,0 +1,226 @@
+// Wave64 / workgroup primitives and the single-pass decoupled-lookback protocol
+// shared by the scan, scan-by-key, copy-if and reduce kernels (CDNA4, gfx950).
+//
+// Memory model.  Tiles publish their aggregate / inclusive prefix in 64-bit
+// "status words" {payload:32 | valid:32}.  A value of any trivially copyable
+// type T is split into ceil(sizeof(T)/4) such words, written with relaxed
+// agent-scope atomic stores and read with relaxed agent-scope atomic loads.
+// Because every word carries its own valid bit, a reader never needs a fence:
+// it only accepts a value once all its words read back valid.  Aggregates and
+// inclusive prefixes live in separate arrays so that a reader never mixes the
+// two.  These atomics are coherent across the 8 XCDs (agent scope), which the
+// per-XCD L2s are not for plain stores.
+#pragma once
+#include "trtc_builtin.hpp"
+#include "trtc_args.h"
+
+namespace trtc {
+
+constexpr int kWave = 64;
+
+TRTC_DEV unsigned lane_id()
+{
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+TRTC_DEV unsigned long long ballot(bool p) { return __ballot(p); }
+
+TRTC_DEV void wave_sync()
+{
+    // Orders this wave's LDS accesses: everything before is visible to every
+    // lane of the wave after.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T> struct Words
+{
+    static constexpr int n = (int)((sizeof(T) + 3) / 4);
+};
+
+// Read lane `src` of a value of any type (word-wise ds_bpermute).
+template <class T> TRTC_DEV T shfl(const T& v, int src)
+{
+    constexpr int W = Words<T>::n;
+    unsigned w[W];
+    w[W - 1] = 0;
+    __builtin_memcpy(w, &v, sizeof(T));
+    unsigned r[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) r[i] = (unsigned)__builtin_amdgcn_ds_bpermute(src << 2, (int)w[i]);
+    T out;
+    __builtin_memcpy(&out, r, sizeof(T));
+    return out;
+}
+
+template <class T> TRTC_DEV T shfl_up(const T& v, unsigned delta)
+{
+    int l = (int)lane_id();
+    int src = l - (int)delta;
+    return shfl(v, src < 0 ? l : src);
+}
+
+template <class T> TRTC_DEV T shfl_down(const T& v, unsigned delta)
+{
+    int l = (int)lane_id();
+    int src = l + (int)delta;
+    return shfl(v, src > 63 ? l : src);
+}
+
+// Inclusive scan across the 64 lanes, earlier lanes always the left operand
+// (correct for any associative op, commutative or not).
+template <class T, class Op> TRTC_DEV T wave_inclusive_scan(T x, Op& op)
+{
+    unsigned l = lane_id();
+#pragma unroll
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        T y = shfl_up(x, d);
+        if (l >= d) x = (T)op(y, x);
+    }
+    return x;
+}
+
+// Segmented variant with validity: lanes whose `valid` is false hold no value;
+// they act as the identity.  Valid lanes always form a prefix of the wave.
+template <class T, class Op> TRTC_DEV T wave_inclusive_scan_partial(T x, bool valid, Op& op)
+{
+    unsigned l = lane_id();
+    unsigned long long vm = ballot(valid);
+#pragma unroll
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        T y = shfl_up(x, d);
+        bool yv = l >= d && ((vm >> (l - d)) & 1ull);
+        if (yv && valid) x = (T)op(y, x);
+    }
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Relaxed agent-scope atomics on status words
+// ---------------------------------------------------------------------------
+TRTC_DEV unsigned long long ld_status(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TRTC_DEV void st_status(unsigned long long* p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TRTC_DEV unsigned ld_status32(const unsigned* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+TRTC_DEV void st_status32(unsigned* p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish value v of tile `tile` in array `arr` (Words<T>::n words per tile).
+template <class T> TRTC_DEV void publish(unsigned long long* arr, size_t tile, const T& v)
+{
+    constexpr int W = Words<T>::n;
+    unsigned w[W];
+    w[W - 1] = 0;
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < W; i++)
+        st_status(arr + tile * W + i, (1ull << 32) | (unsigned long long)w[i]);
+}
+
+// Try to read tile `tile`'s value; true if every word is valid.
+template <class T> TRTC_DEV bool try_read(const unsigned long long* arr, size_t tile, T& out)
+{
+    constexpr int W = Words<T>::n;
+    unsigned w[W];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        unsigned long long s = ld_status(arr + tile * W + i);
+        ok = ok && (s >> 32) != 0;
+        w[i] = (unsigned)s;
+    }
+    __builtin_memcpy(&out, w, sizeof(T));
+    return ok;
+}
+
+// Combine lanes 0..last (lane 0 = nearest predecessor tile, so higher lanes
+// are earlier tiles and go on the left).  Returns the result in every lane.
+template <class T, class Op> TRTC_DEV T wave_reverse_reduce(T x, unsigned last, Op& op)
+{
+    unsigned l = lane_id();
+#pragma unroll
+    for (unsigned d = 1; d < 64; d <<= 1) {
+        T y = shfl_down(x, d);  // lane l + d: an earlier tile
+        if (l + d <= last && (l & (2 * d - 1)) == 0) x = (T)op(y, x);
+    }
+    return shfl(x, 0);
+}
+
+// Decoupled lookback run by one full wave for tile `tile` >= 1: returns the
+// inclusive prefix of all tiles before `tile`.  The window of 64 predecessors
+// is read in parallel (lane k reads tile - 1 - k).  A predecessor's
+// inclusive prefix ends the walk; otherwise its aggregate is folded in and
+// the window moves back by 64.  Every tile publishes its aggregate before it
+// looks back and tile ids are handed out in launch order by an atomic counter,
+// so every awaited tile is already resident and the wait ends.
+template <class T, class Op>
+TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op)
+{
+    unsigned l = lane_id();
+    T acc;
+    bool have = false;
+    long long top = (long long)tile - 1;
+    while (true) {
+        long long j = top - (long long)l;
+        T v;
+        int st = 3;  // 3: before tile 0 (not part of the window)
+        if (j >= 0) {
+            // issue both reads before testing either (one latency, not two)
+            T vi, va;
+            bool oki = try_read<T>(lb.incl, (size_t)j, vi);
+            bool oka = try_read<T>(lb.agg, (size_t)j, va);
+            st = oki ? 2 : (oka ? 1 : 0);
+            v = oki ? vi : va;
+        }
+        unsigned long long incl_mask = ballot(st == 2);
+        unsigned long long wait_mask = ballot(st == 0);
+        unsigned last;
+        if (incl_mask) {
+            last = (unsigned)__builtin_ctzll(incl_mask);
+            unsigned long long below = (last == 63) ? ~0ull : ((2ull << last) - 1);
+            if (wait_mask & below) {
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+        } else {
+            if (wait_mask) {
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            unsigned long long in_window = ballot(st != 3);
+            last = 63 - (unsigned)__builtin_clzll(in_window);
+        }
+        T w = wave_reverse_reduce(v, last, op);
+        acc = have ? (T)op(w, acc) : w;
+        have = true;
+        if (incl_mask) break;
+        top -= 64;
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-local striped <-> blocked transposition through LDS.
+// Striped: lane l owns elements k*64 + l.  Blocked: lane l owns elements
+// l*ITEMS + j.  One pad word every 32 words keeps the blocked accesses
+// bank-conflict-light.
+// ---------------------------------------------------------------------------
+template <int ITEMS> struct WaveTile
+{
+    static constexpr int elems = 64 * ITEMS;
+    static constexpr int padded = elems + elems / 32 + 1;
+    TRTC_DEV static int pad(int i) { return i + (i >> 5); }
+};
+
+}  // namespace trtc

@@ -1,0 +1,511 @@
+// Host drivers of the primitive core: argument checks, kernel selection (an
+// ahead-of-time instantiation when one exists, else hipRTC), scratch sizing and
+// launches.  Entry points keep the reference's C++ API (sort.h, scan.h,
+// reduce.h, copy.h) and defaults (sort.cpp:30-65, scan.cpp:12-39,
+// scan_by_key.cpp:4-53, reduce.cpp:5-31, copy.cpp:16-28).
+#include <algorithm>
+#include <deque>
+
+#include "copy.h"
+#include "reduce.h"
+#include "scan.h"
+#include "sort.h"
+#include "trtc_internal.h"
+#include "../device/trtc_args.h"
+
+using namespace trtc_host;
+
+namespace {
+
+constexpr unsigned kBlock = 256;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Items per thread of the scan-family tiles (must match ScanItems / ReduceItems /
+// CopyIfItems / ScanByKeyItems in the device headers).
+int items_for(size_t elem_size) { return elem_size <= 4 ? 16 : (elem_size <= 8 ? 8 : 4); }
+size_t words_for(size_t bytes) { return (bytes + 3) / 4; }
+
+std::string vcls(const DeviceViewable& v) { return canon_type(v.name_view_cls()); }
+
+// Kernel arguments: each argument's bytes are kept alive until launch.
+struct Args
+{
+    std::deque<ViewBuf> bufs;
+    std::vector<void*> ptrs;
+    void view(const DeviceViewable& v)
+    {
+        bufs.push_back(v.view());
+        ptrs.push_back(bufs.back().data());
+    }
+    void raw(const void* p, size_t n)
+    {
+        bufs.push_back(ViewBuf((const char*)p, (const char*)p + n));
+        ptrs.push_back(bufs.back().data());
+    }
+    template <class T> void val(const T& v) { raw(&v, sizeof(T)); }
+    void** data() { return ptrs.data(); }
+};
+
+// Scalar value of an init DeviceViewable converted to `to` through the
+// reference's cast chain: init -> input element type -> output element type.
+bool init_value(const DeviceViewable* init, const std::string& in_elem, const std::string& out_elem, ViewBuf& out)
+{
+    const ScalarInfo* so = scalar_info(out_elem);
+    size_t out_size = so ? so->size : TRTC_Size_Of(out_elem.c_str());
+    out.assign(out_size, 0);
+    if (!init) return true;  // default: (T)0
+    ViewBuf iv = init->view();
+    const ScalarInfo* si = scalar_info(init->name_view_cls());
+    const ScalarInfo* sm = scalar_info(in_elem);
+    if (si && sm && so) {
+        char mid[8];
+        if (!convert_scalar(si->code, iv.data(), sm->code, mid)) return false;
+        return convert_scalar(sm->code, mid, so->code, out.data());
+    }
+    if (canon_type(init->name_view_cls()) == canon_type(out_elem) && iv.size() >= out_size) {
+        memcpy(out.data(), iv.data(), out_size);
+        return true;
+    }
+    TRTC_ERR("cannot convert init of type %s to %s", init->name_view_cls().c_str(), out_elem.c_str());
+    return false;
+}
+
+struct LookbackScratch
+{
+    PoolBuf buf;
+    unsigned* counter = nullptr;
+    unsigned long long* agg = nullptr;
+    unsigned long long* incl = nullptr;
+    size_t words = 0;
+    bool make(size_t tiles, size_t value_bytes, hipStream_t s)
+    {
+        size_t W = words_for(value_bytes);
+        size_t bytes = 16 + 2 * tiles * W * 8;
+        new (&buf) PoolBuf(bytes, s);
+        if (!buf.p) return false;
+        counter = (unsigned*)buf.p;
+        agg = (unsigned long long*)((char*)buf.p + 16);
+        incl = agg + tiles * W;
+        return fill_u32(buf.p, bytes / 4, 0, s);
+    }
+};
+
+bool radix_elem(const std::string& elem, int& key_bytes, int& kind)
+{
+    const ScalarInfo* si = scalar_info(elem);
+    if (!si) return false;
+    switch (si->code) {
+    case TC_I8: case TC_I16: case TC_I32: case TC_I64: kind = 1; break;
+    case TC_U8: case TC_U16: case TC_U32: case TC_U64: case TC_BOOL: kind = 0; break;
+    case TC_F32: case TC_F64: kind = 2; break;
+    default: return false;
+    }
+    key_bytes = (int)si->size;
+    return true;
+}
+
+// ---------------------------------------------------------------- radix sort
+// Onesweep LSD radix sort of n keys (key_bytes wide) with optional values
+// (val_bytes in {0,1,2,4,8}); result in place.  Stable.
+bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool descending, int val_bytes,
+                hipStream_t s)
+{
+    if (n <= 1) return true;
+    const int KB = key_bytes;
+    const int ITEMS = 16;
+    const size_t TILE = (size_t)kBlock * ITEMS;
+    const size_t tiles = (n + TILE - 1) / TILE;
+    const bool wide = n >= (1ull << 30);
+    const size_t sw = wide ? 8 : 4;
+
+    PoolBuf ktmp(n * KB, s);
+    PoolBuf vtmp(val_bytes ? n * val_bytes : 0, s);
+    // scratch: hist [KB*256 u64] | done u32 (+pad) | tile counters [8 u32] | status [KB][tiles][256] | digit_start
+    size_t off_hist = 0;
+    size_t off_done = off_hist + KB * 256 * 8;
+    size_t off_cnt = off_done + 16;
+    size_t off_status = align_up(off_cnt + 8 * 4, 256);
+    size_t status_bytes_per_pass = align_up(tiles * 256 * sw, 256);
+    size_t off_dstart = off_status + KB * status_bytes_per_pass;
+    size_t scratch_bytes = off_dstart + KB * 256 * 8;
+    PoolBuf scratch(scratch_bytes, s);
+    if (!ktmp.p || (val_bytes && !vtmp.p) || !scratch.p) return false;
+    char* sc = (char*)scratch.p;
+
+    if (!fill_u32(sc, off_cnt / 4, 0, s)) return false;  // hist + done
+
+    KernelRef hk;
+    std::string hname = "trtc_radix_hist_k<" + std::to_string(KB) + ">";
+    if (!get_template_kernel(hname, "", hk)) return false;
+    trtc::RadixHistArgs ha;
+    ha.keys = keys;
+    ha.n = n;
+    ha.kind = kind;
+    ha.descending = descending ? 1 : 0;
+    ha.passes = KB;
+    ha.hist = (unsigned long long*)(sc + off_hist);
+    ha.digit_start = (unsigned long long*)(sc + off_dstart);
+    ha.done = (unsigned*)(sc + off_done);
+    ha.zero_ptr = (unsigned*)(sc + off_cnt);
+    ha.zero_words = (off_dstart - off_cnt) / 4;
+    unsigned hgrid = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, (size_t)std::max(1, context().num_cus) * 4);
+    {
+        void* args[] = {&ha};
+        if (!launch_kernel(hk, dim3(hgrid), dim3(kBlock), args, 0, s)) return false;
+    }
+
+    KernelRef pk;
+    std::string pname = "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
+                        (wide ? "unsigned long long" : "unsigned int") + ">";
+    if (!get_template_kernel(pname, "", pk)) return false;
+    trtc::RadixPassArgs pa;
+    void* kbuf[2] = {keys, ktmp.p};
+    void* vbuf[2] = {vals, vtmp.p};
+    for (int p = 0; p < KB; p++) {
+        pa.keys_in = kbuf[p & 1];
+        pa.keys_out = kbuf[(p + 1) & 1];
+        pa.vals_in = vbuf[p & 1];
+        pa.vals_out = vbuf[(p + 1) & 1];
+        pa.n = n;
+        pa.digit_start = (const unsigned long long*)(sc + off_dstart) + p * 256;
+        pa.status = sc + off_status + p * status_bytes_per_pass;
+        pa.tile_counter = (unsigned*)(sc + off_cnt) + p;
+        pa.shift = 8 * p;
+        pa.kind = kind;
+        pa.descending = descending ? 1 : 0;
+        void* args[] = {&pa};
+        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kBlock), args, 0, s)) return false;
+    }
+    if (KB & 1) {
+        if (!check_hip(hipMemcpyAsync(keys, ktmp.p, n * KB, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+            return false;
+        if (val_bytes &&
+            !check_hip(hipMemcpyAsync(vals, vtmp.p, n * val_bytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+            return false;
+    }
+    return true;
+}
+
+bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp)
+{
+    if (!ensure_init()) return false;
+    const std::string& cn = comp.name_view_cls();
+    int kb = 0, kind = 0;
+    if ((cn == "Less" || cn == "Greater") && radix_elem(keys.name_elem_cls(), kb, kind)) {
+        void* kp = contiguous_data(keys);
+        if (!kp) {
+            TRTC_ERR("Sort: keys must be a DVVector or DVVectorAdaptor");
+            return false;
+        }
+        size_t n = keys.size();
+        hipStream_t s = cur_stream();
+        bool desc = cn == "Greater";
+        if (!values) return radix_sort(kp, nullptr, n, kb, kind, desc, 0, s);
+        void* vp = contiguous_data(*values);
+        if (!vp) {
+            TRTC_ERR("Sort_By_Key: values must be a DVVector or DVVectorAdaptor");
+            return false;
+        }
+        if (values->size() < n) {
+            TRTC_ERR("Sort_By_Key: values (%zu) shorter than keys (%zu)", values->size(), n);
+            return false;
+        }
+        size_t vb = values->elem_size();
+        if (vb == 1 || vb == 2 || vb == 4 || vb == 8) return radix_sort(kp, vp, n, kb, kind, desc, (int)vb, s);
+        // other value sizes: sort an index payload, then gather the values
+        PoolBuf idx(n * 4, s), vcopy(n * vb, s);
+        if (!idx.p || !vcopy.p) return false;
+        if (!iota_u32(idx.as<unsigned>(), n, s)) return false;
+        if (!radix_sort(kp, idx.p, n, kb, kind, desc, 4, s)) return false;
+        if (!check_hip(hipMemcpyAsync(vcopy.p, vp, n * vb, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+            return false;
+        return gather_bytes(vcopy.p, vp, idx.as<unsigned>(), n, vb, s);
+    }
+    TRTC_ERR("Sort: comparator '%s' on element type '%s' is not supported (only Less/Greater on integer and "
+             "floating-point keys)",
+             cn.c_str(), keys.name_elem_cls().c_str());
+    return false;
+}
+
+// ---------------------------------------------------------------- scan
+
+bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op, const DeviceViewable* init,
+                  bool exclusive)
+{
+    if (!ensure_init()) return false;
+    size_t n = vin.size();
+    if (n == 0) return true;
+    if (vout.size() < n) {
+        TRTC_ERR("Scan: output shorter than input");
+        return false;
+    }
+    std::string T = canon_type(vout.name_elem_cls());
+    size_t tsz = vout.elem_size();
+    ViewBuf initv;
+    if (!init_value(init, vin.name_elem_cls(), vout.name_elem_cls(), initv)) return false;
+    std::string name = "trtc_scan_k<" + vcls(vin) + ", " + vcls(vout) + ", " + vcls(op) + ", " + T + ">";
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return false;
+    hipStream_t s = cur_stream();
+    size_t tile = (size_t)kBlock * items_for(tsz);
+    size_t tiles = (n + tile - 1) / tile;
+    LookbackScratch lb;
+    if (!lb.make(tiles, tsz, s)) return false;
+    trtc::ScanArgs a{n, {lb.agg, lb.incl, lb.counter}, exclusive ? 1 : 0};
+    Args args;
+    args.view(vin);
+    args.view(vout);
+    args.view(op);
+    args.raw(initv.data(), initv.size());
+    args.val(a);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s);
+}
+
+// ---------------------------------------------------------------- reduce
+
+bool general_reduce(const DVVectorLike& vec, const DeviceViewable* init, const Functor& op, ViewBuf& ret)
+{
+    if (!ensure_init()) return false;
+    size_t n = vec.size();
+    std::string T = canon_type(vec.name_elem_cls());
+    size_t tsz = vec.elem_size();
+    ViewBuf initv;
+    if (!init_value(init, vec.name_elem_cls(), vec.name_elem_cls(), initv)) return false;
+    ret.assign(tsz, 0);
+    if (n == 0) {
+        if (!init) return false;  // the reference's general_reduce rejects n < 1
+        ret = initv;
+        return true;
+    }
+    std::string name = "trtc_reduce_k<" + vcls(vec) + ", " + vcls(op) + ", " + T + ">";
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return false;
+    hipStream_t s = cur_stream();
+    const size_t waves_per_block = kBlock / 64;
+    const size_t chunk = 64 * (size_t)items_for(tsz);
+    size_t chunks = (n + chunk - 1) / chunk;
+    size_t max_blocks = (size_t)std::max(1, context().num_cus) * 8;
+    size_t blocks = std::min(max_blocks, (chunks + waves_per_block - 1) / waves_per_block);
+    size_t waves = blocks * waves_per_block;
+    size_t per_wave = (chunks + waves - 1) / waves * chunk;
+    blocks = (n + per_wave * waves_per_block - 1) / (per_wave * waves_per_block);
+    // partials: Maybe<T> = {T v; unsigned has} per block
+    size_t maybe_bytes = align_up(tsz + 4, std::min<size_t>(std::max<size_t>(tsz, 4), 8));
+    size_t W = words_for(maybe_bytes);
+    size_t res_off = 16 + blocks * W * 8;
+    PoolBuf scratch(res_off + align_up(tsz, 8), s);
+    if (!scratch.p) return false;
+    if (!fill_u32(scratch.p, res_off / 4, 0, s)) return false;
+    trtc::ReduceArgs a{n, per_wave, (unsigned long long*)((char*)scratch.p + 16), (unsigned*)scratch.p,
+                     (char*)scratch.p + res_off, init ? 1 : 0};
+    Args args;
+    args.view(vec);
+    args.view(op);
+    args.raw(initv.data(), initv.size());
+    args.val(a);
+    if (!launch_kernel(k, dim3((unsigned)blocks), dim3(kBlock), args.data(), 0, s)) return false;
+    if (!check_hip(hipMemcpyAsync(ret.data(), a.result, tsz, hipMemcpyDeviceToHost, s), "hipMemcpyAsync"))
+        return false;
+    return check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------- copy_if
+
+uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVectorLike& vout, const Functor& pred)
+{
+    if (!ensure_init()) return (uint32_t)-1;
+    size_t n = vin.size();
+    if (n == 0) return 0;
+    if (sten && sten->size() < n) {
+        TRTC_ERR("Copy_If_Stencil: stencil shorter than input");
+        return (uint32_t)-1;
+    }
+    const DVVectorLike& st = sten ? *sten : vin;
+    std::string name =
+        "trtc_copy_if_k<" + vcls(vin) + ", " + vcls(st) + ", " + vcls(vout) + ", " + vcls(pred) + ">";
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return (uint32_t)-1;
+    hipStream_t s = cur_stream();
+    size_t tile = (size_t)kBlock * items_for(vin.elem_size());
+    size_t tiles = (n + tile - 1) / tile;
+    LookbackScratch lb;
+    if (!lb.make(tiles, 8, s)) return (uint32_t)-1;
+    PoolBuf cnt(8, s);
+    if (!cnt.p) return (uint32_t)-1;
+    trtc::CopyIfArgs a{n, {lb.agg, lb.incl, lb.counter}, cnt.as<unsigned long long>(), (unsigned)tiles};
+    Args args;
+    args.view(vin);
+    args.view(st);
+    args.view(vout);
+    args.view(pred);
+    args.val(a);
+    if (!launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s)) return (uint32_t)-1;
+    unsigned long long count = 0;
+    if (!check_hip(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return (uint32_t)-1;
+    if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return (uint32_t)-1;
+    if (count >= 0xFFFFFFFFull) {
+        TRTC_ERR("Copy_If: count does not fit the 32-bit return value");
+        return (uint32_t)-1;
+    }
+    return (uint32_t)count;
+}
+
+// ---------------------------------------------------------------- scan_by_key
+
+bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVVectorLike& out, const Functor& pred,
+                         const Functor& op, const DeviceViewable* init, bool exclusive)
+{
+    if (!ensure_init()) return false;
+    size_t n = keys.size();
+    if (n == 0) return true;
+    if (vals.size() < n || out.size() < n) {
+        TRTC_ERR("Scan_By_Key: values/output shorter than keys");
+        return false;
+    }
+    std::string V = canon_type(out.name_elem_cls());
+    size_t vsz = out.elem_size();
+    ViewBuf initv;
+    if (!init_value(init, vals.name_elem_cls(), out.name_elem_cls(), initv)) return false;
+    std::string name = "trtc_scan_by_key_k<" + vcls(keys) + ", " + vcls(vals) + ", " + vcls(out) + ", " + vcls(pred) +
+                       ", " + vcls(op) + ", " + V + ">";
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return false;
+    hipStream_t s = cur_stream();
+    size_t tile = (size_t)kBlock * items_for(vsz);
+    size_t tiles = (n + tile - 1) / tile;
+    size_t seg_bytes = align_up(vsz + 4, std::min<size_t>(std::max<size_t>(vsz, 4), 8));
+    LookbackScratch lb;
+    if (!lb.make(tiles, seg_bytes, s)) return false;
+    trtc::ScanByKeyArgs a{n, {lb.agg, lb.incl, lb.counter}, exclusive ? 1 : 0};
+    Args args;
+    args.view(keys);
+    args.view(vals);
+    args.view(out);
+    args.view(pred);
+    args.view(op);
+    args.raw(initv.data(), initv.size());
+    args.val(a);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s);
+}
+
+}  // namespace
+
+// ================================================================ public API
+bool TRTC_Sort(DVVectorLike& vec, const Functor& comp) { return sort_dispatch(vec, nullptr, comp); }
+bool TRTC_Sort(DVVectorLike& vec) { return TRTC_Sort(vec, Functor("Less")); }
+bool TRTC_Sort_By_Key(DVVectorLike& keys, DVVectorLike& values, const Functor& comp)
+{
+    return sort_dispatch(keys, &values, comp);
+}
+bool TRTC_Sort_By_Key(DVVectorLike& keys, DVVectorLike& values)
+{
+    return TRTC_Sort_By_Key(keys, values, Functor("Less"));
+}
+
+bool TRTC_Is_Sorted(const DVVectorLike& vec, const Functor& comp, bool& result)
+{
+    // reference sort.cpp:67-83
+    result = true;
+    if (vec.size() <= 1) return true;
+    static TRTC_For s_for({"vec", "comp", "res"}, "idx", "    if (comp(vec[idx+1], vec[idx])) res[0] = false;\n");
+    DVVector dvres("bool", 1, &result);
+    const DeviceViewable* args[] = {&vec, &comp, &dvres};
+    if (!s_for.launch_n(vec.size() - 1, args)) return false;
+    dvres.to_host(&result);
+    return true;
+}
+bool TRTC_Is_Sorted(const DVVectorLike& vec, bool& result) { return TRTC_Is_Sorted(vec, Functor("Less"), result); }
+
+bool TRTC_Inclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& binary_op)
+{
+    return general_scan(vec_in, vec_out, binary_op, nullptr, false);
+}
+bool TRTC_Inclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out)
+{
+    return TRTC_Inclusive_Scan(vec_in, vec_out, Functor("Plus"));
+}
+bool TRTC_Exclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out)
+{
+    return general_scan(vec_in, vec_out, Functor("Plus"), nullptr, true);
+}
+bool TRTC_Exclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out, const DeviceViewable& init,
+                         const Functor& binary_op)
+{
+    return general_scan(vec_in, vec_out, binary_op, &init, true);
+}
+bool TRTC_Exclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out, const DeviceViewable& init)
+{
+    return TRTC_Exclusive_Scan(vec_in, vec_out, init, Functor("Plus"));
+}
+
+bool TRTC_Inclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out,
+                                const Functor& binary_pred, const Functor& binary_op)
+{
+    return general_scan_by_key(vec_key, vec_value, vec_out, binary_pred, binary_op, nullptr, false);
+}
+bool TRTC_Inclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out,
+                                const Functor& binary_pred)
+{
+    return TRTC_Inclusive_Scan_By_Key(vec_key, vec_value, vec_out, binary_pred, Functor("Plus"));
+}
+bool TRTC_Inclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out)
+{
+    return TRTC_Inclusive_Scan_By_Key(vec_key, vec_value, vec_out, Functor("EqualTo"));
+}
+bool TRTC_Exclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out)
+{
+    return general_scan_by_key(vec_key, vec_value, vec_out, Functor("EqualTo"), Functor("Plus"), nullptr, true);
+}
+bool TRTC_Exclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out,
+                                const DeviceViewable& init, const Functor& binary_pred, const Functor& binary_op)
+{
+    return general_scan_by_key(vec_key, vec_value, vec_out, binary_pred, binary_op, &init, true);
+}
+bool TRTC_Exclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out,
+                                const DeviceViewable& init, const Functor& binary_pred)
+{
+    return TRTC_Exclusive_Scan_By_Key(vec_key, vec_value, vec_out, init, binary_pred, Functor("Plus"));
+}
+bool TRTC_Exclusive_Scan_By_Key(const DVVectorLike& vec_key, const DVVectorLike& vec_value, DVVectorLike& vec_out,
+                                const DeviceViewable& init)
+{
+    return TRTC_Exclusive_Scan_By_Key(vec_key, vec_value, vec_out, init, Functor("EqualTo"));
+}
+
+bool TRTC_Reduce(const DVVectorLike& vec, ViewBuf& ret) { return general_reduce(vec, nullptr, Functor("Plus"), ret); }
+bool TRTC_Reduce(const DVVectorLike& vec, const DeviceViewable& init, const Functor& binary_op, ViewBuf& ret)
+{
+    return general_reduce(vec, &init, binary_op, ret);
+}
+bool TRTC_Reduce(const DVVectorLike& vec, const DeviceViewable& init, ViewBuf& ret)
+{
+    return general_reduce(vec, &init, Functor("Plus"), ret);
+}
+
+uint32_t TRTC_Copy_If(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& pred)
+{
+    return general_copy_if(vec_in, nullptr, vec_out, pred);
+}
+uint32_t TRTC_Copy_If_Stencil(const DVVectorLike& vec_in, const DVVectorLike& vec_stencil, DVVectorLike& vec_out,
+                              const Functor& pred)
+{
+    return general_copy_if(vec_in, &vec_stencil, vec_out, pred);
+}
+
+// Element-wise converting copy (reference copy.cpp:3-12).
+bool TRTC_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out)
+{
+    if (!ensure_init()) return false;
+    size_t n = vec_in.size();
+    if (n == 0) return true;
+    void* src = contiguous_data(vec_in);
+    void* dst = contiguous_data(vec_out);
+    if (src && dst && canon_type(vec_in.name_elem_cls()) == canon_type(vec_out.name_elem_cls()))
+        return check_hip(hipMemcpyAsync(dst, src, n * vec_in.elem_size(), hipMemcpyDeviceToDevice, cur_stream()),
+                         "hipMemcpyAsync");
+    static TRTC_For s_for({"view_vec_in", "view_vec_out"}, "idx",
+                          "    view_vec_out[idx] = (decltype(view_vec_out)::value_t)view_vec_in[idx];\n");
+    const DeviceViewable* args[] = {&vec_in, &vec_out};
+    return s_for.launch_n(n, args);
+}

@@ -1,0 +1,396 @@
+// C ABI (include/trtc_capi.h): thin wrappers over the C++ API with the
+// reference's conventions (python/api_*.cpp): handles are heap objects, a NULL
+// functor / init selects the default overload, ints return 0 / -1 / -2.
+#include <algorithm>
+#include <cstring>
+
+#include "copy.h"
+#include "reduce.h"
+#include "scan.h"
+#include "sort.h"
+#include "trtc_capi.h"
+#include "trtc_internal.h"
+
+using namespace trtc_host;
+
+typedef std::vector<std::string> StrArray;
+typedef std::vector<const DeviceViewable*> PtrArray;
+
+// ---- utils
+void* n_string_array_create(unsigned long long size, const char* const* strs)
+{
+    StrArray* r = new StrArray(size);
+    for (size_t i = 0; i < size; i++) (*r)[i] = strs[i];
+    return r;
+}
+void n_string_array_destroy(void* p) { delete (StrArray*)p; }
+void* n_pointer_array_create(unsigned long long size, const void* const* ptrs)
+{
+    PtrArray* r = new PtrArray(size);
+    if (size) memcpy(r->data(), ptrs, sizeof(void*) * size);
+    return r;
+}
+void n_pointer_array_destroy(void* p) { delete (PtrArray*)p; }
+void* n_dim3_create(unsigned x, unsigned y, unsigned z) { return new dim_type{x, y, z}; }
+void n_dim3_destroy(void* p) { delete (dim_type*)p; }
+
+// ---- context
+void n_set_libnvrtc_path(const char* path) { set_libnvrtc_path(path); }
+int n_trtc_try_init(void) { return TRTC_Try_Init() ? 0 : -100; }
+int n_get_ptx_arch(void) { return TRTC_Get_PTX_Arch(); }
+void n_set_verbose(unsigned v) { TRTC_Set_Verbose(v != 0); }
+void n_set_kernel_debug(unsigned v) { TRTC_Set_Kernel_Debug(v != 0); }
+void n_add_include_dir(const char* dir) { TRTC_Add_Include_Dir(dir); }
+void n_add_built_in_header(const char* fn, const char* content) { TRTC_Add_Built_In_Header(fn, content); }
+void n_add_inlcude_filename(const char* fn) { TRTC_Add_Inlcude_Filename(fn); }
+void n_add_code_block(const char* line) { TRTC_Add_Code_Block(line); }
+void n_add_constant_object(const char* name, void* cptr) { TRTC_Add_Constant_Object(name, *(DeviceViewable*)cptr); }
+void n_wait(void) { TRTC_Wait(); }
+
+static std::vector<const char*> cstrs(const StrArray& a)
+{
+    std::vector<const char*> r(a.size());
+    for (size_t i = 0; i < a.size(); i++) r[i] = a[i].c_str();
+    return r;
+}
+
+void* n_kernel_create(void* ptr_param_list, const char* body)
+{
+    return new TRTC_Kernel(cstrs(*(StrArray*)ptr_param_list), body);
+}
+void n_kernel_destroy(void* cptr) { delete (TRTC_Kernel*)cptr; }
+int n_kernel_num_params(void* cptr) { return (int)((TRTC_Kernel*)cptr)->num_params(); }
+
+static bool arg_count_ok(size_t want, const PtrArray* args)
+{
+    if (args->size() == want) return true;
+    printf("Wrong number of arguments received. %d required, %d received.", (int)want, (int)args->size());
+    return false;
+}
+
+int n_kernel_calc_optimal_block_size(void* ptr_kernel, void* ptr_arg_list, unsigned sharedMemBytes)
+{
+    TRTC_Kernel* k = (TRTC_Kernel*)ptr_kernel;
+    PtrArray* a = (PtrArray*)ptr_arg_list;
+    if (!arg_count_ok(k->num_params(), a)) return -2;
+    int s;
+    if (k->calc_optimal_block_size(a->data(), s, sharedMemBytes)) return s;
+    return -1;
+}
+
+int n_kernel_calc_number_blocks(void* ptr_kernel, void* ptr_arg_list, int sizeBlock, unsigned sharedMemBytes)
+{
+    TRTC_Kernel* k = (TRTC_Kernel*)ptr_kernel;
+    PtrArray* a = (PtrArray*)ptr_arg_list;
+    if (!arg_count_ok(k->num_params(), a)) return -2;
+    int nb;
+    if (k->calc_number_blocks(a->data(), sizeBlock, nb, sharedMemBytes)) return nb;
+    return -1;
+}
+
+int n_kernel_launch(void* ptr_kernel, void* ptr_gridDim, void* ptr_blockDim, void* ptr_arg_list, int sharedMemBytes)
+{
+    TRTC_Kernel* k = (TRTC_Kernel*)ptr_kernel;
+    PtrArray* a = (PtrArray*)ptr_arg_list;
+    if (!arg_count_ok(k->num_params(), a)) return -2;
+    return k->launch(*(dim_type*)ptr_gridDim, *(dim_type*)ptr_blockDim, a->data(), (unsigned)sharedMemBytes) ? 0 : -1;
+}
+
+void* n_for_create(void* ptr_param_list, const char* name_iter, const char* body)
+{
+    return new TRTC_For(cstrs(*(StrArray*)ptr_param_list), name_iter, body);
+}
+void n_for_destroy(void* cptr) { delete (TRTC_For*)cptr; }
+int n_for_num_params(void* cptr) { return (int)((TRTC_For*)cptr)->num_params(); }
+int n_for_launch(void* cptr, int begin, int end, void* ptr_arg_list)
+{
+    TRTC_For* f = (TRTC_For*)cptr;
+    PtrArray* a = (PtrArray*)ptr_arg_list;
+    if (!arg_count_ok(f->num_params(), a)) return -2;
+    return f->launch((size_t)begin, (size_t)end, a->data()) ? 0 : -1;
+}
+int n_for_launch_n(void* cptr, int n, void* ptr_arg_list)
+{
+    TRTC_For* f = (TRTC_For*)cptr;
+    PtrArray* a = (PtrArray*)ptr_arg_list;
+    if (!arg_count_ok(f->num_params(), a)) return -2;
+    return f->launch_n((size_t)n, a->data()) ? 0 : -1;
+}
+
+// ---- device-viewable scalars
+const char* n_dv_name_view_cls(void* cptr) { return ((DeviceViewable*)cptr)->name_view_cls().c_str(); }
+void n_dv_destroy(void* cptr) { delete (DeviceViewable*)cptr; }
+
+template <class T> static T dv_value(void* cptr)
+{
+    ViewBuf b = ((DeviceViewable*)cptr)->view();
+    T v;
+    memcpy(&v, b.data(), sizeof(T));
+    return v;
+}
+
+void* n_dvint8_create(int v) { return new DVInt8((int8_t)v); }
+int n_dvint8_value(void* c) { return dv_value<int8_t>(c); }
+void* n_dvuint8_create(unsigned v) { return new DVUInt8((uint8_t)v); }
+unsigned n_dvuint8_value(void* c) { return dv_value<uint8_t>(c); }
+void* n_dvint16_create(int v) { return new DVInt16((int16_t)v); }
+int n_dvint16_value(void* c) { return dv_value<int16_t>(c); }
+void* n_dvuint16_create(unsigned v) { return new DVUInt16((uint16_t)v); }
+unsigned n_dvuint16_value(void* c) { return dv_value<uint16_t>(c); }
+void* n_dvint32_create(int v) { return new DVInt32(v); }
+int n_dvint32_value(void* c) { return dv_value<int32_t>(c); }
+void* n_dvuint32_create(unsigned v) { return new DVUInt32(v); }
+unsigned n_dvuint32_value(void* c) { return dv_value<uint32_t>(c); }
+void* n_dvint64_create(long long v) { return new DVInt64(v); }
+long long n_dvint64_value(void* c) { return dv_value<int64_t>(c); }
+void* n_dvuint64_create(unsigned long long v) { return new DVUInt64(v); }
+unsigned long long n_dvuint64_value(void* c) { return dv_value<uint64_t>(c); }
+void* n_dvfloat_create(float v) { return new DVFloat(v); }
+float n_dvfloat_value(void* c) { return dv_value<float>(c); }
+void* n_dvdouble_create(double v) { return new DVDouble(v); }
+double n_dvdouble_value(void* c) { return dv_value<double>(c); }
+void* n_dvbool_create(int v) { return new DVBool(v != 0); }
+int n_dvbool_value(void* c) { return dv_value<bool>(c) ? 1 : 0; }
+void* n_dvcomplex64_create(float re, float im) { return new DVComplex64(re, im); }
+float n_dvcomplex64_real(void* c) { return dv_value<float>(c); }
+float n_dvcomplex64_imag(void* c)
+{
+    ViewBuf b = ((DeviceViewable*)c)->view();
+    float v;
+    memcpy(&v, b.data() + 4, 4);
+    return v;
+}
+void* n_dvcomplex128_create(double re, double im) { return new DVComplex128(re, im); }
+double n_dvcomplex128_real(void* c) { return dv_value<double>(c); }
+double n_dvcomplex128_imag(void* c)
+{
+    ViewBuf b = ((DeviceViewable*)c)->view();
+    double v;
+    memcpy(&v, b.data() + 8, 8);
+    return v;
+}
+
+// ---- vectors
+const char* n_dvvectorlike_name_elem_cls(void* cptr) { return ((DVVectorLike*)cptr)->name_elem_cls().c_str(); }
+unsigned long long n_dvvectorlike_size(void* cptr) { return ((DVVectorLike*)cptr)->size(); }
+
+void* n_dvvector_create(const char* elem_cls, unsigned long long size, void* hdata)
+{
+    DVVector* v = new DVVector(elem_cls, size, hdata);
+    if (size > 0 && !v->data()) {
+        delete v;
+        return nullptr;
+    }
+    return v;
+}
+
+void n_dvvector_to_host(void* cptr, void* hdata, unsigned long long begin, unsigned long long end)
+{
+    ((DVVector*)cptr)->to_host(hdata, begin, end);
+}
+
+void* n_dvvector_from_dvs(void* ptr_dvs)
+{
+    PtrArray* dvs = (PtrArray*)ptr_dvs;
+    size_t n = dvs->size();
+    if (n < 1) return nullptr;
+    std::string elem = (*dvs)[0]->name_view_cls();
+    for (size_t i = 1; i < n; i++)
+        if ((*dvs)[i]->name_view_cls() != elem) return nullptr;
+    size_t es = (*dvs)[0]->view().size();
+    std::vector<char> buf(es * n);
+    for (size_t i = 0; i < n; i++) memcpy(buf.data() + es * i, (*dvs)[i]->view().data(), es);
+    return n_dvvector_create(elem.c_str(), n, buf.data());
+}
+
+void* n_dvvectoradaptor_create(const char* elem_cls, unsigned long long size, void* data)
+{
+    return new DVVectorAdaptor(elem_cls, size, data);
+}
+
+void* n_dvrange_create(void* ptr_in, unsigned long long begin, unsigned long long end)
+{
+    DVVectorLike* v = (DVVectorLike*)ptr_in;
+    if (auto p = dynamic_cast<DVVector*>(v)) return new DVVectorAdaptor(*p, begin, end);
+    if (auto p = dynamic_cast<DVVectorAdaptor*>(v)) return new DVVectorAdaptor(*p, begin, end);
+    return new DVRange(*v, begin, end);
+}
+
+void* n_dvvectorlike_data(void* cptr) { return contiguous_data(*(DVVectorLike*)cptr); }
+
+// ---- functors
+void* n_functor_create(void* ptr_dvs, void* ptr_names, void* ptr_functor_params, const char* code_body)
+{
+    PtrArray* dvs = (PtrArray*)ptr_dvs;
+    StrArray* names = (StrArray*)ptr_names;
+    std::vector<CapturedDeviceViewable> arg_map(dvs->size());
+    for (size_t i = 0; i < dvs->size(); i++) {
+        arg_map[i].obj_name = (*names)[i].c_str();
+        arg_map[i].obj = (*dvs)[i];
+    }
+    return new Functor(arg_map, cstrs(*(StrArray*)ptr_functor_params), code_body);
+}
+
+void* n_built_in_functor_create(const char* name) { return new Functor(name); }
+
+// ---- algorithms
+int n_fill(void* ptr_vec, void* ptr_value)
+{
+    // reference fill.cpp: vec[idx] = value for every element
+    static TRTC_For s_for({"view_vec", "value"}, "idx",
+                          "    view_vec[idx] = (decltype(view_vec)::value_t)value;\n");
+    DVVectorLike* vec = (DVVectorLike*)ptr_vec;
+    const DeviceViewable* args[] = {vec, (DeviceViewable*)ptr_value};
+    return s_for.launch_n(vec->size(), args) ? 0 : -1;
+}
+
+int n_copy(void* ptr_in, void* ptr_out)
+{
+    return TRTC_Copy(*(DVVectorLike*)ptr_in, *(DVVectorLike*)ptr_out) ? 0 : -1;
+}
+
+void* n_reduce(void* ptr_vec, void* ptr_init, void* ptr_bin_op)
+{
+    DVVectorLike* vec = (DVVectorLike*)ptr_vec;
+    ViewBuf ret;
+    bool ok;
+    if (!ptr_init) ok = TRTC_Reduce(*vec, ret);
+    else if (!ptr_bin_op) ok = TRTC_Reduce(*vec, *(DeviceViewable*)ptr_init, ret);
+    else ok = TRTC_Reduce(*vec, *(DeviceViewable*)ptr_init, *(Functor*)ptr_bin_op, ret);
+    if (!ok) return nullptr;
+    return dv_from_viewbuf(ret, vec->name_elem_cls().c_str());
+}
+
+int n_is_sorted(void* ptr_vec, void* ptr_comp)
+{
+    bool res;
+    bool ok = ptr_comp ? TRTC_Is_Sorted(*(DVVectorLike*)ptr_vec, *(Functor*)ptr_comp, res)
+                       : TRTC_Is_Sorted(*(DVVectorLike*)ptr_vec, res);
+    if (!ok) return -1;
+    return res ? 1 : 0;
+}
+
+int n_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_binary_op)
+{
+    DVVectorLike* in = (DVVectorLike*)ptr_vec_in;
+    DVVectorLike* out = (DVVectorLike*)ptr_vec_out;
+    bool ok = ptr_binary_op ? TRTC_Inclusive_Scan(*in, *out, *(Functor*)ptr_binary_op) : TRTC_Inclusive_Scan(*in, *out);
+    return ok ? 0 : -1;
+}
+
+int n_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_init, void* ptr_binary_op)
+{
+    DVVectorLike* in = (DVVectorLike*)ptr_vec_in;
+    DVVectorLike* out = (DVVectorLike*)ptr_vec_out;
+    bool ok;
+    if (!ptr_init) ok = TRTC_Exclusive_Scan(*in, *out);
+    else if (!ptr_binary_op) ok = TRTC_Exclusive_Scan(*in, *out, *(DeviceViewable*)ptr_init);
+    else ok = TRTC_Exclusive_Scan(*in, *out, *(DeviceViewable*)ptr_init, *(Functor*)ptr_binary_op);
+    return ok ? 0 : -1;
+}
+
+int n_inclusive_scan_by_key(void* ptr_vec_key, void* ptr_vec_value, void* ptr_vec_out, void* ptr_binary_pred,
+                            void* ptr_binary_op)
+{
+    DVVectorLike* k = (DVVectorLike*)ptr_vec_key;
+    DVVectorLike* v = (DVVectorLike*)ptr_vec_value;
+    DVVectorLike* o = (DVVectorLike*)ptr_vec_out;
+    bool ok;
+    if (!ptr_binary_pred) ok = TRTC_Inclusive_Scan_By_Key(*k, *v, *o);
+    else if (!ptr_binary_op) ok = TRTC_Inclusive_Scan_By_Key(*k, *v, *o, *(Functor*)ptr_binary_pred);
+    else ok = TRTC_Inclusive_Scan_By_Key(*k, *v, *o, *(Functor*)ptr_binary_pred, *(Functor*)ptr_binary_op);
+    return ok ? 0 : -1;
+}
+
+int n_exclusive_scan_by_key(void* ptr_vec_key, void* ptr_vec_value, void* ptr_vec_out, void* ptr_init,
+                            void* ptr_binary_pred, void* ptr_binary_op)
+{
+    DVVectorLike* k = (DVVectorLike*)ptr_vec_key;
+    DVVectorLike* v = (DVVectorLike*)ptr_vec_value;
+    DVVectorLike* o = (DVVectorLike*)ptr_vec_out;
+    bool ok;
+    if (!ptr_init) ok = TRTC_Exclusive_Scan_By_Key(*k, *v, *o);
+    else if (!ptr_binary_pred) ok = TRTC_Exclusive_Scan_By_Key(*k, *v, *o, *(DeviceViewable*)ptr_init);
+    else if (!ptr_binary_op)
+        ok = TRTC_Exclusive_Scan_By_Key(*k, *v, *o, *(DeviceViewable*)ptr_init, *(Functor*)ptr_binary_pred);
+    else
+        ok = TRTC_Exclusive_Scan_By_Key(*k, *v, *o, *(DeviceViewable*)ptr_init, *(Functor*)ptr_binary_pred,
+                                        *(Functor*)ptr_binary_op);
+    return ok ? 0 : -1;
+}
+
+unsigned n_copy_if(void* ptr_in, void* ptr_out, void* ptr_pred)
+{
+    return TRTC_Copy_If(*(DVVectorLike*)ptr_in, *(DVVectorLike*)ptr_out, *(Functor*)ptr_pred);
+}
+
+unsigned n_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_pred)
+{
+    return TRTC_Copy_If_Stencil(*(DVVectorLike*)ptr_in, *(DVVectorLike*)ptr_stencil, *(DVVectorLike*)ptr_out,
+                                *(Functor*)ptr_pred);
+}
+
+int n_sort(void* ptr_vec, void* ptr_comp)
+{
+    DVVectorLike* v = (DVVectorLike*)ptr_vec;
+    bool ok = ptr_comp ? TRTC_Sort(*v, *(Functor*)ptr_comp) : TRTC_Sort(*v);
+    return ok ? 0 : -1;
+}
+
+int n_sort_by_key(void* ptr_keys, void* ptr_values, void* ptr_comp)
+{
+    DVVectorLike* k = (DVVectorLike*)ptr_keys;
+    DVVectorLike* v = (DVVectorLike*)ptr_values;
+    bool ok = ptr_comp ? TRTC_Sort_By_Key(*k, *v, *(Functor*)ptr_comp) : TRTC_Sort_By_Key(*k, *v);
+    return ok ? 0 : -1;
+}
+
+// ---- extensions
+int n_segmented_sort_by_key(void* ptr_keys, void* ptr_values, const unsigned long long* offsets,
+                            unsigned long long num_segments, void* ptr_comp)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor less("Less");
+    const Functor& comp = ptr_comp ? *(Functor*)ptr_comp : less;
+    return TRTC_Segmented_Sort_By_Key(*(DVVectorLike*)ptr_keys, *(DVVectorLike*)ptr_values, off.data(), num_segments,
+                                      comp)
+               ? 0
+               : -1;
+}
+
+int n_segmented_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
+                               unsigned long long num_segments)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    return TRTC_Segmented_Inclusive_Scan(*(DVVectorLike*)ptr_vec_in, *(DVVectorLike*)ptr_vec_out, off.data(),
+                                         num_segments)
+               ? 0
+               : -1;
+}
+
+void n_set_stream(void* stream) { TRTC_Set_Stream(stream); }
+void* n_get_stream(void) { return TRTC_Get_Stream(); }
+
+int n_jit_compile_check(const char* arch, const char* name_expr)
+{
+    std::string log;
+    if (compile_only(arch, name_expr, name_expr, &log)) return 0;
+    return -1;
+}
+
+int n_profile_begin(void)
+{
+    profile_begin();
+    return 0;
+}
+
+unsigned long long n_profile_end(char* json_out, unsigned long long capacity)
+{
+    std::string js = profile_end();
+    if (json_out && capacity) {
+        size_t n = std::min<size_t>(js.size(), capacity - 1);
+        memcpy(json_out, js.data(), n);
+        json_out[n] = 0;
+    }
+    return js.size();
+}

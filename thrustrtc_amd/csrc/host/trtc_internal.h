@@ -1,0 +1,146 @@
+// Internal interfaces of the host runtime (not installed).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "DVVector.h"
+#include "TRTCContext.h"
+#include "functor.h"
+
+namespace trtc_host {
+
+// ---------------------------------------------------------------- types
+enum TypeCode {
+    TC_I8 = 0, TC_U8, TC_I16, TC_U16, TC_I32, TC_U32, TC_I64, TC_U64, TC_F32, TC_F64, TC_BOOL, TC_OTHER
+};
+
+struct ScalarInfo
+{
+    const char* canon;  // canonical device type name (e.g. "int32_t")
+    TypeCode code;
+    size_t size;
+};
+
+// Scalar type info for any spelling the reference accepts ("int", "int32_t",
+// "unsigned long long", "size_t", ...); nullptr for non-scalars.
+const ScalarInfo* scalar_info(const std::string& name);
+// Canonical spelling of a type name: scalars are mapped to their fixed-width
+// name, VectorView<...> / RangeView<...> arguments are canonicalised too.
+std::string canon_type(const std::string& name);
+// Numeric conversion between scalar types with C++ cast semantics.
+bool convert_scalar(TypeCode from, const void* src, TypeCode to, void* dst);
+const char* type_name(TypeCode tc);
+size_t type_size(TypeCode tc);
+
+// ---------------------------------------------------------------- layout
+// Alignment of the device type named view_cls whose view has view_size bytes.
+size_t view_align(const std::string& view_cls, size_t view_size);
+// Size/alignment of registered structs (functors).
+void register_struct_layout(const std::string& name, size_t size, size_t align);
+
+// ---------------------------------------------------------------- context
+struct Context
+{
+    bool ok = false;
+    int device = 0;
+    std::string arch;      // full target id, e.g. gfx950:sramecc+:xnack-
+    std::string arch_base; // gfx950
+    int arch_num = 0;      // 950
+    int num_cus = 0;
+    size_t lds_per_block = 0;
+    bool verbose = false;
+    bool kernel_debug = false;
+
+    std::mutex mtx;
+    std::vector<std::string> include_dirs;
+    std::vector<std::string> header_names;
+    std::vector<std::string> header_contents;
+    std::vector<std::string> code_blocks;
+    std::vector<std::pair<std::string, ViewBuf>> constants;
+};
+
+Context& context();
+// Initialise the process-wide context (once) and make its device current on
+// this thread.  False if no usable GPU.
+bool ensure_init();
+hipStream_t cur_stream();
+void set_stream(hipStream_t s);
+
+bool check_hip(hipError_t e, const char* what);
+
+// ---------------------------------------------------------------- structs
+// Register a struct body; returns its name _S_<hash>.  Thread-safe.
+std::string add_struct(const std::string& body);
+// Definitions (in registration order) of every registered struct named in
+// `text`, including the structs those definitions name in turn.
+std::string struct_defs_for(const std::string& text);
+
+// ---------------------------------------------------------------- memory
+// Stream-ordered caching pool for temporaries: blocks are reused only on the
+// stream that freed them.
+void* pool_alloc(size_t bytes, hipStream_t s);
+void pool_free(void* p, hipStream_t s);
+
+struct PoolBuf
+{
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    PoolBuf() {}
+    PoolBuf(size_t bytes, hipStream_t st) : s(st) { p = bytes ? pool_alloc(bytes, st) : nullptr; }
+    ~PoolBuf() { if (p) pool_free(p, s); }
+    PoolBuf(const PoolBuf&) = delete;
+    PoolBuf& operator=(const PoolBuf&) = delete;
+    template <class T> T* as() const { return (T*)p; }
+};
+
+// ---------------------------------------------------------------- kernels
+struct KernelRef
+{
+    const void* aot = nullptr;       // host stub of an ahead-of-time kernel
+    hipFunction_t jit = nullptr;     // function of a run-time compiled module
+    const char* name = "";           // name expression (stable storage)
+    bool valid() const { return aot || jit; }
+};
+
+// Ahead-of-time kernels register themselves under their name expression.
+void register_aot(const char* name_expr, const void* fn);
+const void* find_aot(const std::string& name_expr);
+
+// A kernel template instantiation by name expression: the ahead-of-time one if
+// it exists, otherwise compiled by hipRTC (cached in memory and on disk).
+// `type_text` names every type involved (used to pull in struct definitions).
+bool get_template_kernel(const std::string& name_expr, const std::string& type_text, KernelRef& out);
+// An extern "C" kernel named `fn_name` defined in `kernel_src` (user kernels).
+bool get_source_kernel(const std::string& kernel_src, const std::string& fn_name, KernelRef& out);
+// Compile only (no device needed): for offline cache warm-up and checks.
+bool compile_only(const std::string& arch, const std::string& name_expr, const std::string& type_text,
+                  std::string* log);
+
+bool launch_kernel(const KernelRef& k, dim3 grid, dim3 block, void** args, size_t shmem, hipStream_t s);
+bool sync_if_debug(hipStream_t s, const char* what);
+// Kernel timing: while enabled, every launch is bracketed by HIP events on its
+// stream; profile_end() synchronises and returns per-kernel counts and times.
+void profile_begin();
+std::string profile_end();
+
+// Built-in utility kernels.
+bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s);
+bool iota_u32(unsigned* p, size_t n, hipStream_t s);
+bool gather_bytes(const void* src, void* dst, const unsigned* idx, size_t n, size_t elem, hipStream_t s);
+
+// ---------------------------------------------------------------- vectors
+// Raw device pointer of a contiguous vector-like (DVVector, DVVectorAdaptor),
+// nullptr otherwise.
+void* contiguous_data(const DVVectorLike& v);
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull);
+
+}  // namespace trtc_host
+
+#define TRTC_ERR(...) (fprintf(stderr, "[ThrustRTC] " __VA_ARGS__), fprintf(stderr, "\n"))
