@@ -37,6 +37,7 @@ struct CopyIfArgs
     LookbackArgs lb;
     unsigned long long* count;  // out: total kept (written by the last tile)
     unsigned num_tiles;
+    int has_stencil;            // predicate reads the stencil, not the input
 };
 
 struct ScanByKeyArgs

@@ -22,9 +22,9 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     constexpr size_t TILE = (size_t)BLOCK * ITEMS;
     typedef typename Sten::value_t ST;
     typedef unsigned long long C;
-    constexpr bool STENCIL = !__is_same(In, Sten);
-    __shared__ T s_x[WAVES][WaveTile<ITEMS>::padded];
-    __shared__ ST s_s[STENCIL ? WAVES : 1][STENCIL ? WaveTile<ITEMS>::padded : 1];
+    constexpr int ESZ = sizeof(T) > sizeof(ST) ? sizeof(T) : sizeof(ST);
+    // one transpose buffer per wave, used for the input, then the stencil
+    __shared__ __attribute__((aligned(16))) unsigned char s_raw[WAVES][WaveTile<ITEMS>::padded * ESZ];
     __shared__ C s_wtot[WAVES];
     __shared__ C s_prefix;
     __shared__ unsigned s_tile;
@@ -38,16 +38,13 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     const size_t tile_base = tile * TILE;
     const size_t wave_base = tile_base + (size_t)wave * WELEMS;
     const bool full = tile_base + TILE <= n;
-    T* lds = s_x[wave];
-    ST* ldss = s_s[STENCIL ? wave : 0];
+    T* lds = (T*)s_raw[wave];
+    ST* ldss = (ST*)s_raw[wave];
 
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         int i = k * 64 + (int)lane;
-        if (full || wave_base + i < n) {
-            lds[WaveTile<ITEMS>::pad(i)] = (T)in[wave_base + i];
-            if (STENCIL) ldss[WaveTile<ITEMS>::pad(i)] = sten[wave_base + i];
-        }
+        if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)in[wave_base + i];
     }
     wave_sync();
     int nvalid = ITEMS;
@@ -56,18 +53,35 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
         nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
     }
     T x[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++)
+        if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
     unsigned flags = 0;
     unsigned cnt = 0;
+    if (a.has_stencil) {
+        wave_sync();
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        if (j < nvalid) {
-            int i = WaveTile<ITEMS>::pad((int)lane * ITEMS + j);
-            x[j] = lds[i];
-            bool f;
-            if (STENCIL) f = (bool)pred(ldss[i]);
-            else f = (bool)pred(x[j]);
-            flags |= (unsigned)f << j;
-            cnt += f;
+        for (int k = 0; k < ITEMS; k++) {
+            int i = k * 64 + (int)lane;
+            if (full || wave_base + i < n) ldss[WaveTile<ITEMS>::pad(i)] = sten[wave_base + i];
+        }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            if (j < nvalid) {
+                bool f = (bool)pred(ldss[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)]);
+                flags |= (unsigned)f << j;
+                cnt += f;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            if (j < nvalid) {
+                bool f = (bool)pred(x[j]);
+                flags |= (unsigned)f << j;
+                cnt += f;
+            }
         }
     }
     Plus plus;
@@ -102,7 +116,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     // pack the kept elements of the tile into LDS (tile-local order), then
     // write them out as one contiguous run
     __syncthreads();  // every wave is done reading its transpose buffer
-    T* pack = &s_x[0][0];
+    T* pack = (T*)&s_raw[0][0];
     unsigned local = (unsigned)(wpre + lexcl);
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
@@ -110,7 +124,9 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     }
     __syncthreads();
     typedef typename Out::value_t OutT;
-    for (unsigned i = threadIdx.x; i < (unsigned)tot; i += BLOCK) out[P + i] = (OutT)pack[i];
+    const size_t cap = out.size();  // never write past the output vector
+    for (unsigned i = threadIdx.x; i < (unsigned)tot; i += BLOCK)
+        if (P + i < cap) out[P + i] = (OutT)pack[i];
 }
 
 template <class T> struct CopyIfItems

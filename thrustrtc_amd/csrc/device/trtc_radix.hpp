@@ -298,7 +298,9 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
         U sk = sort_key<U>(kb, a.kind, flip);
         unsigned d = (unsigned)(sk >> a.shift) & 255u;
         gpos[r] = s_base[d] + j;
-        if (full || j < valid) kout[gpos[r]] = kb;
+        // gpos < n holds by construction; the check keeps a logic error from
+        // ever writing outside the output
+        if ((full || j < valid) && gpos[r] < (P)n) kout[gpos[r]] = kb;
     }
     if (VB) {
         __syncthreads();
@@ -309,7 +311,7 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
             unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
-            if (full || j < valid) vout[gpos[r]] = s_vals[j];
+            if ((full || j < valid) && gpos[r] < (P)n) vout[gpos[r]] = s_vals[j];
         }
     }
 }

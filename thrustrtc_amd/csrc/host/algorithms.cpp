@@ -333,7 +333,8 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
     if (!lb.make(tiles, 8, s)) return (uint32_t)-1;
     PoolBuf cnt(8, s);
     if (!cnt.p) return (uint32_t)-1;
-    trtc::CopyIfArgs a{n, {lb.agg, lb.incl, lb.counter}, cnt.as<unsigned long long>(), (unsigned)tiles};
+    trtc::CopyIfArgs a{n, {lb.agg, lb.incl, lb.counter}, cnt.as<unsigned long long>(), (unsigned)tiles,
+                       sten ? 1 : 0};
     Args args;
     args.view(vin);
     args.view(st);
