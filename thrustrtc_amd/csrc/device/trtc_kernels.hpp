@@ -77,10 +77,16 @@ __global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_hist_k(trtc::RadixHistA
     trtc::radix_hist<TRTC_BLOCK, KB>(a);
 }
 
-#define TRTC_RADIX_ITEMS 16
+// Radix digit pass: 512 threads; 16 keys per thread for keys and values of
+// up to 4 bytes (8192-key tiles), 8 per thread when either is 8 bytes.
+#define TRTC_RADIX_BLOCK 512
+template <int KB, int VB> struct RadixItems
+{
+    static constexpr int value = (KB > 4 || VB > 4) ? 8 : 16;
+};
 
 template <int KB, int VB, class S>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a)
+__global__ void __launch_bounds__(TRTC_RADIX_BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a)
 {
-    trtc::radix_pass<TRTC_BLOCK, TRTC_RADIX_ITEMS, KB, VB, S>(a);
+    trtc::radix_pass<TRTC_RADIX_BLOCK, RadixItems<KB, VB>::value, KB, VB, S>(a);
 }

@@ -79,33 +79,60 @@ TRTC_DEV void st_relaxed(unsigned long long* p, unsigned long long v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One read of the keys for all digit histograms.  Also clears the passes'
-// lookback state (grid-stride), so no separate clearing launch is needed.
+// One read of the keys for all digit histograms (per-wave private LDS
+// histograms, 16-byte nontemporal loads).  Also clears the passes' lookback
+// state (grid-stride), so no separate clearing launch is needed.  The last
+// workgroup to finish turns the global histograms into exclusive digit offsets.
 template <int BLOCK, int KB>
 TRTC_DEV void radix_hist(const RadixHistArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
-    __shared__ unsigned s_h[KB][256];
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int PER_VEC = 16 / KB;  // keys per 16-byte load
+    __shared__ unsigned s_h[WAVES][KB][256];
     __shared__ bool s_last;
-    for (int i = threadIdx.x; i < KB * 256; i += BLOCK) (&s_h[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < WAVES * KB * 256; i += BLOCK) (&s_h[0][0][0])[i] = 0;
     __syncthreads();
 
+    const unsigned wave = threadIdx.x / 64;
     const U flip = a.descending ? (U)~(U)0 : (U)0;
     const U* keys = (const U*)a.keys;
     const size_t stride = (size_t)gridDim.x * BLOCK;
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < a.n; i += stride) {
+    const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t done_keys = 0;
+    if ((((size_t)keys) & 15) == 0) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const v4u* kv = (const v4u*)keys;
+        const size_t nvec = a.n / PER_VEC;
+        for (size_t v = gid; v < nvec; v += stride) {
+            v4u x = __builtin_nontemporal_load(kv + v);
+            U ks[PER_VEC];
+            __builtin_memcpy(ks, &x, 16);
+#pragma unroll
+            for (int e = 0; e < PER_VEC; e++) {
+                U k = sort_key<U>(ks[e], a.kind, flip);
+#pragma unroll
+                for (int p = 0; p < KB; p++)
+                    __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        done_keys = nvec * PER_VEC;
+    }
+    for (size_t i = done_keys + gid; i < a.n; i += stride) {
         U k = sort_key<U>(keys[i], a.kind, flip);
 #pragma unroll
         for (int p = 0; p < KB; p++)
-            __hip_atomic_fetch_add(&s_h[p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    for (size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x; i < a.zero_words; i += stride)
-        a.zero_ptr[i] = 0;
+    for (size_t i = gid; i < a.zero_words; i += stride) a.zero_ptr[i] = 0;
     __syncthreads();
 
     for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
-        unsigned c = (&s_h[0][0])[i];
+        unsigned c = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) c += (&s_h[w][0][0])[i];
         if (c) __hip_atomic_fetch_add(a.hist + i, (unsigned long long)c, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -134,6 +161,29 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     }
 }
 
+// Barrier that orders LDS only: outstanding global loads and stores stay in
+// flight across it (a full __syncthreads would wait for all of them).
+TRTC_DEV void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <class T> TRTC_DEV T stream_load(const T* p) { return __builtin_nontemporal_load(p); }
+
+// One digit pass.  Tile = BLOCK*ITEMS keys; item k of a lane sits at tile
+// position wave*64*ITEMS + k*64 + lane (coalesced striped loads), and ranks
+// are assigned in that position order, which keeps the pass stable.
+//   1. load keys and values (values stay in flight until they are staged),
+//   2. count digits in LDS and publish the tile's 256 counts at once
+//      ("aggregate"; the first tile publishes them as inclusive prefixes),
+//   3. rank inside each wave with 8 ballots per key,
+//   4. issue the first lookback reads, then do the local digit scan and stage
+//      the keys in LDS in digit order while those reads are in flight,
+//   5. finish the lookback (one thread per digit, LBB predecessors per read),
+//      publish the inclusive prefix,
+//   6. write keys, then values, each digit run to consecutive addresses.
 template <int BLOCK, int ITEMS, int KB, int VB, class S>
 TRTC_DEV void radix_pass(const RadixPassArgs& a)
 {
@@ -142,21 +192,24 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int STAGE_BYTES = TILE * (KB > VB ? KB : VB);
+    constexpr int LBB = 4;
     typedef StatusBits<S> SB;
     typedef S P;  // positions fit the status count field
     static_assert(BLOCK >= 256, "one thread per digit");
 
     __shared__ unsigned s_wcnt[WAVES][256];       // per-wave digit counts -> tile offsets
+    __shared__ unsigned s_hist[256];              // tile digit counts
     __shared__ P s_base[256];                     // global position of local slot 0 per digit
-    __shared__ unsigned s_dscan[WAVES];
+    __shared__ unsigned s_dscan[4];
     __shared__ unsigned s_tile;
     __shared__ __attribute__((aligned(16))) unsigned char s_stage[STAGE_BYTES];
     U* s_keys = (U*)s_stage;
     V* s_vals = (V*)s_stage;
 
     for (int i = threadIdx.x; i < WAVES * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < 256; i += BLOCK) s_hist[i] = 0;
     if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-    __syncthreads();
+    lds_barrier();
 
     const size_t tile = s_tile;
     const unsigned wave = threadIdx.x / 64;
@@ -166,147 +219,165 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     const bool full = tile_base + TILE <= n;
     const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
     const U flip = a.descending ? (U)~(U)0 : (U)0;
-    const U* kin = (const U*)a.keys_in;
-    const V* vin = (const V*)a.vals_in;
+    const U* kin = (const U*)a.keys_in + tile_base;
+    const V* vin = (const V*)a.vals_in + tile_base;
     const int wpos0 = (int)wave * 64 * ITEMS;
 
-    // 1. striped loads: item k of this lane sits at tile position wpos0 + k*64 + lane
+    // 1. loads
     U key[ITEMS];
     V val[ITEMS];
     unsigned dig[ITEMS];
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-        if (full || p < valid) {
-            key[k] = kin[tile_base + p];
-            if (VB) val[k] = vin[tile_base + p];
+        for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (wpos0 + k * 64) + lane);
+        if (VB) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (wpos0 + k * 64) + lane);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+            if (p < valid) {
+                key[k] = kin[p];
+                if (VB) val[k] = vin[p];
+            }
         }
     }
+    // 2. digits (positions past the end rank as the largest sort key, digit
+    //    255, and are never written) and the tile histogram
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-        // positions past the end rank as the largest sort key (digit 255) and
-        // are never written
         U sk = (full || p < valid) ? sort_key<U>(key[k], a.kind, flip) : (U)~(U)0;
         dig[k] = (unsigned)(sk >> a.shift) & 255u;
+        __hip_atomic_fetch_add(&s_hist[dig[k]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-
-    // 2. in-wave ranking: peers = lanes with the same digit in this slot
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    unsigned rank[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const unsigned d = dig[k];
-        unsigned long long peers = ~0ull;
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const unsigned long long bal = ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        const unsigned leader = (unsigned)__builtin_ctzll(peers);
-        const unsigned cnt = (unsigned)__builtin_popcountll(peers);
-        unsigned old = 0;
-        if (lane == leader)
-            old = __hip_atomic_fetch_add(&s_wcnt[wave][d], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        old = (unsigned)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
-        rank[k] = old + (unsigned)__builtin_popcountll(peers & lt_mask);
-    }
-    __syncthreads();
-
-    // 3. per digit (thread d): wave offsets, tile count, tile-local digit start
-    unsigned tcount = 0;
+    lds_barrier();
+    S* st = (S*)a.status;
+    const bool head = tile == 0;
+    unsigned real = 0;
     if (threadIdx.x < 256) {
         const unsigned d = threadIdx.x;
-        unsigned run = 0;
-#pragma unroll
-        for (int w = 0; w < WAVES; w++) {
-            unsigned c = s_wcnt[w][d];
-            s_wcnt[w][d] = run;
-            run += c;
-        }
-        tcount = run;
-        // publish this tile's real count (padding keys all sit in digit 255)
-        unsigned real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
-        S* st = (S*)a.status;
-        if (tile == 0) st_relaxed(st + d, (S)(SB::incl | (S)real));
-        else st_relaxed(st + tile * 256 + d, (S)(SB::agg | (S)real));
+        unsigned c = s_hist[d];
+        real = (d == 255 && !full) ? c - (unsigned)(TILE - valid) : c;
+        st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
     }
-    // exclusive scan of tcount over the 256 digits (threads 0..255 = 4 waves)
+
+    // 3. in-wave ranking: peers = lanes with the same digit in this slot;
+    //    the rank is packed above the digit (rank < 64*ITEMS)
+    {
+        const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const unsigned d = dig[k];
+            unsigned long long peers = ~0ull;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long bal = ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const unsigned leader = (unsigned)__builtin_ctzll(peers);
+            const unsigned cnt = (unsigned)__builtin_popcountll(peers);
+            unsigned old = 0;
+            if (lane == leader)
+                old = __hip_atomic_fetch_add(&s_wcnt[wave][d], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            old = (unsigned)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
+            dig[k] |= (old + (unsigned)__builtin_popcountll(peers & lt_mask)) << 8;
+        }
+    }
+    lds_barrier();
+
+    // 4. first lookback reads in flight; local digit scan
+    S w[LBB];
     unsigned dstart = 0;
     if (threadIdx.x < 256) {
+        const unsigned d = threadIdx.x;
+#pragma unroll
+        for (int b = 0; b < LBB; b++) {
+            long long j = (long long)tile - 1 - b;
+            w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + d) : (S)0;
+        }
+        unsigned run = 0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) {
+            unsigned c = s_wcnt[ww][d];
+            s_wcnt[ww][d] = run;
+            run += c;
+        }
         Plus plus;
-        unsigned inc = wave_inclusive_scan(tcount, plus);
+        unsigned inc = wave_inclusive_scan(run, plus);
         if (lane == 63) s_dscan[wave] = inc;
-        dstart = inc - tcount;
+        dstart = inc - run;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < 256) {
         const unsigned d = threadIdx.x;
-        for (unsigned w = 0; w < wave; w++) dstart += s_dscan[w];
+        for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
 #pragma unroll
-        for (int w = 0; w < WAVES; w++) s_wcnt[w][d] += dstart;
+        for (int ww = 0; ww < WAVES; ww++) s_wcnt[ww][d] += dstart;
+    }
+    lds_barrier();
+    unsigned lpos[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        lpos[k] = s_wcnt[wave][dig[k] & 255u] + (dig[k] >> 8);
+        s_keys[lpos[k]] = key[k];
+    }
 
-        // 4. lookback over earlier tiles for digit d
+    // 5. finish the lookback for digit d
+    if (threadIdx.x < 256) {
+        const unsigned d = threadIdx.x;
         unsigned long long excl = 0;
-        if (tile > 0) {
-            S* st = (S*)a.status;
+        if (!head) {
             long long j = (long long)tile - 1;
+            bool first = true;
             while (true) {
-                // read up to 4 predecessors at once
-                S w0 = ld_relaxed(st + (size_t)j * 256 + d);
-                S w1 = j >= 1 ? ld_relaxed(st + (size_t)(j - 1) * 256 + d) : (S)0;
-                S w2 = j >= 2 ? ld_relaxed(st + (size_t)(j - 2) * 256 + d) : (S)0;
-                S w3 = j >= 3 ? ld_relaxed(st + (size_t)(j - 3) * 256 + d) : (S)0;
-                S ws[4] = {w0, w1, w2, w3};
+                S ws[LBB];
+#pragma unroll
+                for (int b = 0; b < LBB; b++)
+                    ws[b] = first ? w[b] : ((j - b >= 0) ? ld_relaxed(st + (size_t)(j - b) * 256 + d) : (S)0);
+                first = false;
                 bool done = false;
                 int used = 0;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (done || used != q) break;
-                    S w = ws[q];
-                    if ((w & ~SB::mask) == 0) break;  // not yet published
-                    excl += (unsigned long long)(w & SB::mask);
-                    used = q + 1;
-                    if (w & SB::incl) done = true;
+                for (int b = 0; b < LBB; b++) {
+                    if (done || used != b) break;
+                    S x = ws[b];
+                    if ((x & ~SB::mask) == 0) break;  // not yet published
+                    excl += (unsigned long long)(x & SB::mask);
+                    used = b + 1;
+                    if (x & SB::incl) done = true;
                 }
                 if (done) break;
                 if (used == 0) __builtin_amdgcn_s_sleep(1);
                 j -= used;
             }
-            unsigned real = (d == 255 && !full) ? tcount - (unsigned)(TILE - valid) : tcount;
             st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
         }
         s_base[d] = (P)(a.digit_start[d] + excl - dstart);
     }
-    __syncthreads();
+    lds_barrier();
 
-    // 5. stage keys in digit order, then write each run to its global slot
-    unsigned lpos[ITEMS];
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        lpos[k] = s_wcnt[wave][dig[k]] + rank[k];
-        s_keys[lpos[k]] = key[k];
-    }
-    __syncthreads();
+    // 6. write each digit run to its global slot
     U* kout = (U*)a.keys_out;
     P gpos[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
         unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
         U kb = s_keys[j];
-        U sk = sort_key<U>(kb, a.kind, flip);
-        unsigned d = (unsigned)(sk >> a.shift) & 255u;
+        unsigned d = (unsigned)(sort_key<U>(kb, a.kind, flip) >> a.shift) & 255u;
         gpos[r] = s_base[d] + j;
         // gpos < n holds by construction; the check keeps a logic error from
         // ever writing outside the output
         if ((full || j < valid) && gpos[r] < (P)n) kout[gpos[r]] = kb;
     }
     if (VB) {
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) s_vals[lpos[k]] = val[k];
-        __syncthreads();
+        lds_barrier();
         V* vout = (V*)a.vals_out;
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {

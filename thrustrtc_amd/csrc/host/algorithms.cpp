@@ -113,8 +113,10 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
 {
     if (n <= 1) return true;
     const int KB = key_bytes;
-    const int ITEMS = 16;
-    const size_t TILE = (size_t)kBlock * ITEMS;
+    // tile shape of trtc_radix_pass_k (RadixItems in the device header)
+    const unsigned kRadixBlock = 512;
+    const int ITEMS = (KB > 4 || val_bytes > 4) ? 8 : 16;
+    const size_t TILE = (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
     const bool wide = n >= (1ull << 30);
     const size_t sw = wide ? 8 : 4;
@@ -175,7 +177,7 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         pa.kind = kind;
         pa.descending = descending ? 1 : 0;
         void* args[] = {&pa};
-        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kBlock), args, 0, s)) return false;
+        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kRadixBlock), args, 0, s)) return false;
     }
     if (KB & 1) {
         if (!check_hip(hipMemcpyAsync(keys, ktmp.p, n * KB, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
