@@ -33,6 +33,9 @@ def main():
     ap.add_argument("--n", type=int, default=28, help="log2 of the number of pairs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--mode", choices=["sort", "copy"], default="sort",
+                    help="copy: a plain device copy of a known size (4 B/elem read + 4 B/elem write), "
+                         "used to calibrate HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         log("bench.py measures one GPU only: the primitive core runs on a single MI355X "
@@ -50,6 +53,22 @@ def main():
 
     N = 1 << a.n
     K, W = a.steps, a.warmup
+
+    if a.mode == "copy":
+        src = trtc.device_vector("uint32_t", N)
+        dst = trtc.device_vector("uint32_t", N)
+        loop = trtc.For(["src", "dst"], "i", "    dst[i] = src[i];\n")
+        for _ in range(W):
+            loop.launch_n(N, [src, dst])
+        trtc.Wait()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            loop.launch_n(N, [src, dst])
+        trtc.Wait()
+        sec = (time.perf_counter() - t0) / K
+        print(json.dumps({"mode": "copy", "n": N, "bytes_per_launch": 8 * N, "ms": round(sec * 1e3, 4),
+                          "gbs": round(8 * N / sec / 1e9, 1)}), flush=True)
+        return
 
     def make_copies(keys, vals, count):
         src_k = trtc.device_vector_from_numpy(keys)
@@ -113,6 +132,16 @@ def main():
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
     model_gbs = 68.0 * N / sec / 1e9
 
+    # HBM bytes per launch of the same kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy kernel of known size);
+    # only valid for the workload they were measured on
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "radix_pass_traffic.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("n") == N and pass_name.startswith(t.get("kernel", "?")[:-1]):
+            traffic = round(t["read_bytes"] + t["write_bytes"])
+
     variants = {}
     if not a.no_variants:
         for name, gen in [("u32_low_entropy", lambda: wl.u32_low_entropy(N, 2)),
@@ -157,7 +186,7 @@ def main():
                                f"+ uint32 original-index payload, ascending (Less), stable",
                    "n": N, "key_bits": 32, "value_bits": 32, "digit_bits": 8, "passes": 4},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": pass_name, "kernel_avg_ms": round(pass_ms, 4),
                      "bytes_per_launch": pass_bytes},
         "model_68N": {"gbs": round(model_gbs, 1), "pct_hbm": round(100 * model_gbs / HBM_PEAK_GBS, 2)},
