@@ -25,6 +25,291 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def golden():
+    """Hashes of the oracle's outputs at the full workload sizes (written by
+    tests/golden/make_large_hashes.py); None when absent."""
+    p = os.path.join(ROOT, "tests", "golden", "large_hashes.json")
+    return json.load(open(p)) if os.path.exists(p) else None
+
+
+def kernel_ms(prof, prefix):
+    names = [k for k in prof if k.startswith(prefix)]
+    if not names:
+        return None, None
+    k = names[0]
+    return k, prof[k]["total_ms"] / prof[k]["count"]
+
+
+def profiled(native, fn, reps):
+    """Runs fn() reps times between n_profile_begin/end; returns (wall seconds
+    per call, per-kernel launch times)."""
+    import ctypes
+    native.n_profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    native.n_wait()
+    dt = (time.perf_counter() - t0) / reps
+    buf = ctypes.create_string_buffer(1 << 16)
+    native.n_profile_end(buf, 1 << 16)
+    return dt, json.loads(buf.value.decode())
+
+
+def leg_copy(trtc, n, reps, warmup):
+    """Device copy of n uint32 (4 B read + 4 B written per element) through a
+    JIT For kernel: the measured HBM ceiling beside the 8 TB/s nominal peak."""
+    src = trtc.device_vector("uint32_t", n)
+    dst = trtc.device_vector("uint32_t", n)
+    loop = trtc.For(["src", "dst"], "i", "    dst[i] = src[i];\n")
+    for _ in range(warmup):
+        loop.launch_n(n, [src, dst])
+    trtc.Wait()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loop.launch_n(n, [src, dst])
+    trtc.Wait()
+    sec = (time.perf_counter() - t0) / reps
+    return {"n": n, "bytes_per_launch": 8 * n, "ms": round(sec * 1e3, 4), "gbs": round(8 * n / sec / 1e9, 1)}
+
+
+def leg_single_pass(trtc, native, wl, a, log):
+    """Inclusive_Scan int32, Reduce float32 and Copy_If int32 (x % 3 == 0,
+    hipRTC functor) on 2^29 elements, each checked against the oracle's
+    full-size output hash."""
+    import numpy as np
+    n = 1 << 29
+    reps = 10
+    gold = golden()
+    res = {}
+    x = wl.i32_range(n, 5, -1000, 1000)
+    dx = trtc.device_vector_from_numpy(x)
+    dout = trtc.device_vector("int32_t", n)
+
+    def h(arr):
+        return "%016x" % wl._lib.wl_fnv1a64(arr.ctypes.data, arr.nbytes)
+
+    # --- Inclusive_Scan
+    trtc.Inclusive_Scan(dx, dout)
+    trtc.Wait()
+    dt, prof = profiled(native, lambda: trtc.Inclusive_Scan(dx, dout), reps)
+    kname, kms = kernel_ms(prof, "trtc_scan_k<")
+    ok = None
+    if gold:
+        ok = h(dout.to_host()) == gold["inclusive_scan_i32_2p29_seed5"]["out"]
+    res["inclusive_scan_i32"] = {"ms": round(dt * 1e3, 4), "gbs_8N": round(8 * n / dt / 1e9, 1),
+                                 "kernel": kname, "kernel_ms": round(kms, 4),
+                                 "kernel_gbs": round(8 * n / kms / 1e6, 1),
+                                 "frac_hbm": round(8 * n / kms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": ok}
+    log(f"scan: {res['inclusive_scan_i32']}")
+
+    # --- Copy_If with a hipRTC functor
+    pred = trtc.Functor({}, ["x"], "        return x % 3 == 0;\n")
+    t0 = time.perf_counter()
+    cnt = trtc.Copy_If(dx, dout, pred)
+    first = time.perf_counter() - t0
+    dt, prof = profiled(native, lambda: trtc.Copy_If(dx, dout, pred), reps)
+    kname, kms = kernel_ms(prof, "trtc_copy_if_k<")
+    algo = 4 * n + 4 * cnt
+    ok = None
+    if gold:
+        g = gold["copy_if_mod3_i32_2p29_seed5"]
+        ok = cnt == g["count"] and h(dout.to_host(0, cnt)) == g["out"]
+    res["copy_if_i32_mod3"] = {"ms": round(dt * 1e3, 4), "kept": int(cnt), "algorithmic_bytes": algo,
+                               "kernel": kname, "kernel_ms": round(kms, 4),
+                               "kernel_gbs": round(algo / kms / 1e6, 1),
+                               "frac_hbm": round(algo / kms / 1e6 / HBM_PEAK_GBS, 4),
+                               "first_call_ms_incl_jit": round(first * 1e3, 1), "bit_exact": ok}
+    log(f"copy_if: {res['copy_if_i32_mod3']}")
+    del dx, dout
+
+    # --- Reduce float32
+    f = wl.f32_unit(n, 6)
+    df = trtc.device_vector_from_numpy(f)
+    trtc.Reduce(df)
+    r = [None]
+
+    def red():
+        r[0] = trtc.Reduce(df)
+    dt, prof = profiled(native, red, reps)
+    kname, kms = kernel_ms(prof, "trtc_reduce_k<")
+    rel = None
+    if gold:
+        ref = gold["reduce_f32_2p29_seed6"]["sum_fp64"]
+        rel = abs(float(r[0]) - ref) / abs(ref)
+    res["reduce_f32"] = {"ms": round(dt * 1e3, 4), "gbs_4N": round(4 * n / dt / 1e9, 1),
+                         "kernel": kname, "kernel_ms": round(kms, 4),
+                         "kernel_gbs": round(4 * n / kms / 1e6, 1),
+                         "frac_hbm": round(4 * n / kms / 1e6 / HBM_PEAK_GBS, 4),
+                         "rel_err_vs_fp64": rel, "within_1e-5": None if rel is None else rel <= 1e-5}
+    log(f"reduce: {res['reduce_f32']}")
+    del df
+
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import Oracle
+        L = Oracle().L
+        out = np.empty_like(x)
+        t0 = time.perf_counter()
+        L.bl_inclusive_scan_i32(x.ctypes.data, out.ctypes.data, n)
+        t_scan = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        L.bl_copy_if_mod3_i32(x.ctypes.data, out.ctypes.data, n)
+        t_cif = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        L.bl_reduce_f32(f.ctypes.data, n)
+        t_red = time.perf_counter() - t0
+        res["cpu"] = {"cores": int(L.bl_threads()), "kind": "port",
+                      "inclusive_scan_gbs_8N": round(8 * n / t_scan / 1e9, 2),
+                      "copy_if_gbs": round((4 * n + 4 * res["copy_if_i32_mod3"]["kept"]) / t_cif / 1e9, 2),
+                      "reduce_gbs_4N": round(4 * n / t_red / 1e9, 2),
+                      "sample": "the full 2^29 inputs, one call each (__gnu_parallel::partial_sum, "
+                                "OpenMP count/scan/write copy_if, __gnu_parallel::accumulate)"}
+        log(f"cpu single-pass: {res['cpu']}")
+    return res
+
+
+def leg_small_calls(trtc, native, wl, log):
+    """Latency of single small calls: TRTC_Sort on 2^20 uint32 (seed 0) and
+    TRTC_Inclusive_Scan on 2^20 int32 in [-100, 100] (seed 0); median over
+    100 calls, input restored by a device copy outside the timed call."""
+    import numpy as np
+    n = 1 << 20
+    gold = golden()
+    k = wl.u32_uniform(n, 0)
+    src = trtc.device_vector_from_numpy(k)
+    dk = trtc.device_vector("uint32_t", n)
+    lat = []
+    for i in range(110):
+        trtc.Copy(src, dk)
+        trtc.Wait()
+        t0 = time.perf_counter()
+        trtc.Sort(dk)
+        trtc.Wait()
+        if i >= 10:
+            lat.append(time.perf_counter() - t0)
+    out = dk.to_host()
+    ok = None
+    if gold:
+        ok = "%016x" % wl._lib.wl_fnv1a64(out.ctypes.data, out.nbytes) == gold["sort_u32_2p20_uniform_seed0"]["keys_out"]
+    res = {"sort_u32": {"median_us": round(1e6 * float(np.median(lat)), 1),
+                        "p99_us": round(1e6 * float(np.percentile(lat, 99)), 1), "bit_exact": ok}}
+    x = wl.i32_range(n, 0, -100, 100)
+    dx = trtc.device_vector_from_numpy(x)
+    dy = trtc.device_vector("int32_t", n)
+    lat = []
+    for i in range(110):
+        t0 = time.perf_counter()
+        trtc.Inclusive_Scan(dx, dy)
+        trtc.Wait()
+        if i >= 10:
+            lat.append(time.perf_counter() - t0)
+    out = dy.to_host()
+    ok = None
+    if gold:
+        ok = "%016x" % wl._lib.wl_fnv1a64(out.ctypes.data, out.nbytes) == gold["inclusive_scan_i32_2p20_seed0"]["out"]
+    res["inclusive_scan_i32"] = {"median_us": round(1e6 * float(np.median(lat)), 1),
+                                 "p99_us": round(1e6 * float(np.percentile(lat, 99)), 1), "bit_exact": ok}
+    log(f"small calls: {res}")
+    return res
+
+
+def leg_segmented(trtc, native, wl, a, log):
+    """Batched segments: 4096 segments with sizes log-uniform in [16, 2^20]
+    (mt19937_64 seed 7), uint32 keys (uniform, seed 2) + uint32 index payload
+    Sort_By_Key and int32 (uniform [-1000, 1000], seed 5) Inclusive_Scan over
+    the same segments, recorded in one Batch (autotuned tile schedule) and
+    replayed as one hipGraph launch; against a loop of one Sort_By_Key and one
+    Inclusive_Scan call per segment through the C ABI."""
+    import ctypes
+    import numpy as np
+    sizes, total = wl.segment_sizes(a.segments, 7, 16, 1 << 20)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    keys = wl.u32_uniform(total, 2)
+    x = wl.i32_range(total, 5, -1000, 1000)
+    k0 = trtc.device_vector_from_numpy(keys)
+    iota = trtc.device_vector_from_numpy(np.arange(total, dtype=np.uint32))
+    dk = trtc.device_vector("uint32_t", total)
+    dv = trtc.device_vector("uint32_t", total)
+    dx = trtc.device_vector_from_numpy(x)
+    dy = trtc.device_vector("int32_t", total)
+
+    def restore():
+        trtc.Copy(k0, dk)
+        trtc.Copy(iota, dv)
+        trtc.Wait()
+
+    def hsh(v):
+        arr = v.to_host()
+        return "%016x" % wl._lib.wl_fnv1a64(arr.ctypes.data, arr.nbytes)
+
+    t0 = time.perf_counter()
+    b = trtc.Batch()
+    b.add_sort_by_key(dk, dv, offs)
+    b.add_inclusive_scan(dx, dy, offs)
+    restore()
+    b.instantiate()
+    inst_s = time.perf_counter() - t0
+    reps = 5
+    walls, gpu = [], []
+    for i in range(reps + 1):
+        restore()
+        t1 = time.perf_counter()
+        dt, prof = profiled(native, b.launch, 1)
+        if i:
+            walls.append(dt)
+            gpu.append(prof.get("trtc_batch_graph", {"total_ms": dt * 1e3})["total_ms"])
+    batched_ms = float(np.median(gpu))
+    hk, hv, hy = hsh(dk), hsh(dv), hsh(dy)
+
+    # per-segment loop through the C ABI (views created outside the timing)
+    kp, vp = native.n_dvvectorlike_data(dk.m_cptr), native.n_dvvectorlike_data(dv.m_cptr)
+    xp, yp = native.n_dvvectorlike_data(dx.m_cptr), native.n_dvvectorlike_data(dy.m_cptr)
+    views = []
+    for s in range(len(sizes)):
+        lo, n = int(offs[s]), int(sizes[s])
+        views.append((trtc.DVVectorAdaptor("uint32_t", n, kp + 4 * lo), trtc.DVVectorAdaptor("uint32_t", n, vp + 4 * lo),
+                      trtc.DVVectorAdaptor("int32_t", n, xp + 4 * lo), trtc.DVVectorAdaptor("int32_t", n, yp + 4 * lo)))
+    loop_ms = []
+    for i in range(3):
+        restore()
+        trtc.Fill(dy, trtc.DVInt32(0))
+        trtc.Wait()
+        t1 = time.perf_counter()
+        for (ka, va, xa, ya) in views:
+            native.n_sort_by_key(ka.m_cptr, va.m_cptr, None)
+            native.n_inclusive_scan(xa.m_cptr, ya.m_cptr, None)
+        trtc.Wait()
+        if i:
+            loop_ms.append((time.perf_counter() - t1) * 1e3)
+    same = (hsh(dk), hsh(dv), hsh(dy)) == (hk, hv, hy)
+    loop = float(np.median(loop_ms))
+
+    # oracle spot check of 32 segments (smallest, largest and a spread)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import Oracle
+    orc = Oracle()
+    gk, gv, gy = dk.to_host(), dv.to_host(), dy.to_host()
+    order = np.argsort(sizes)
+    pick = sorted(set(order[:8].tolist() + order[-8:].tolist() + order[:: max(1, len(order) // 16)].tolist()))
+    spot = True
+    for s in pick:
+        lo, hi = int(offs[s]), int(offs[s + 1])
+        perm = orc.sort_perm(keys[lo:hi], "uint32_t")
+        spot = spot and bool((gk[lo:hi] == keys[lo:hi][perm]).all()) and bool((gv[lo:hi] == perm + lo).all())
+        spot = spot and bool((gy[lo:hi] == orc.inclusive_scan(x[lo:hi], "int32_t")).all())
+    res = {"segments": int(len(sizes)), "total_elems": int(total),
+           "sizes": "log-uniform [16, 2^20], mt19937_64 seed 7",
+           "batched_graph_ms": round(batched_ms, 3), "batched_wall_ms": round(1e3 * float(np.median(walls)), 3),
+           "per_segment_loop_ms": round(loop, 3), "speedup_vs_loop": round(loop / batched_ms, 2),
+           "sort_gkeys_per_s_incl_scan": round(total / batched_ms / 1e6, 2),
+           "pct_hbm_68N_plus_8N": round(100 * (76.0 * total) / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+           "instantiate_s_incl_tuning": round(inst_s, 2),
+           "batch": b.describe(), "tuning": trtc.Batch_Tuning_Report(),
+           "batched_equals_loop_bitwise": same, "oracle_spot_check_32_segments": spot}
+    log(f"segmented: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,6 +318,12 @@ def main():
     ap.add_argument("--n", type=int, default=28, help="log2 of the number of pairs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-single-pass", action="store_true", help="skip the 2^29 scan / reduce / copy_if leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the 2^20 latency leg")
+    ap.add_argument("--no-segmented", action="store_true", help="skip the batched-segments leg")
+    ap.add_argument("--segments", type=int, default=4096, help="segments of the batched-segments leg")
+    ap.add_argument("--only", choices=["sort", "single_pass", "small", "segmented"], default=None,
+                    help="run one leg only (the headline fields are then absent)")
     ap.add_argument("--mode", choices=["sort", "copy"], default="sort",
                     help="copy: a plain device copy of a known size (4 B/elem read + 4 B/elem write), "
                          "used to calibrate HBM byte counters")
@@ -55,19 +346,17 @@ def main():
     K, W = a.steps, a.warmup
 
     if a.mode == "copy":
-        src = trtc.device_vector("uint32_t", N)
-        dst = trtc.device_vector("uint32_t", N)
-        loop = trtc.For(["src", "dst"], "i", "    dst[i] = src[i];\n")
-        for _ in range(W):
-            loop.launch_n(N, [src, dst])
-        trtc.Wait()
-        t0 = time.perf_counter()
-        for _ in range(K):
-            loop.launch_n(N, [src, dst])
-        trtc.Wait()
-        sec = (time.perf_counter() - t0) / K
-        print(json.dumps({"mode": "copy", "n": N, "bytes_per_launch": 8 * N, "ms": round(sec * 1e3, 4),
-                          "gbs": round(8 * N / sec / 1e9, 1)}), flush=True)
+        print(json.dumps(dict(mode="copy", **leg_copy(trtc, N, K, W))), flush=True)
+        return
+
+    if a.only and a.only != "sort":
+        if a.only == "single_pass":
+            res = leg_single_pass(trtc, native, wl, a, log)
+        elif a.only == "small":
+            res = leg_small_calls(trtc, native, wl, log)
+        else:
+            res = leg_segmented(trtc, native, wl, a, log)
+        print(json.dumps({"leg": a.only, "result": res}), flush=True)
         return
 
     def make_copies(keys, vals, count):
@@ -169,6 +458,12 @@ def main():
                          f"(same keys), {dt:.2f} s"}
         log(f"cpu baseline: {cpu}")
 
+    segmented = None if a.no_segmented else leg_segmented(trtc, native, wl, a, log)
+    single_pass = None if a.no_single_pass else leg_single_pass(trtc, native, wl, a, log)
+    small = None if a.no_small else leg_small_calls(trtc, native, wl, log)
+    copy_bw = leg_copy(trtc, N, 5, 2)
+    log(f"copy bandwidth: {copy_bw}")
+
     line = {
         "metric": "Gkeys/s Sort_By_Key 2^28 u32 + %HBM roofline; served-mix p99 vs per-call",
         "value": round(gkeys, 4),
@@ -193,6 +488,10 @@ def main():
         "kernels": prof,
         "variants": variants,
         "cpu_baseline": cpu,
+        "single_pass_2p29": single_pass,
+        "batched_segments": segmented,
+        "small_calls_2p20": small,
+        "device_copy": copy_bw,
     }
     print(json.dumps(line), flush=True)
 

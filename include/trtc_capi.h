@@ -114,8 +114,9 @@ TRTC_CAPI int n_sort(void* ptr_vec, void* ptr_comp);                  /* api.h:1
 TRTC_CAPI int n_sort_by_key(void* ptr_keys, void* ptr_values, void* ptr_comp); /* api.h:190 */
 
 /* ---- extensions: no counterpart in the reference ----
- * Batched segmented primitives and the request server.  Callers: the batched
- * benchmark and services that submit many independent sorts / scans. */
+ * Batched segmented primitives (one launch per kernel stage for many
+ * independent segments) and the graph-captured batch.  Callers: workloads
+ * with many independent sorts / scans of uneven sizes. */
 /* Sort each segment [offsets[s], offsets[s+1]) of keys/values independently
  * (offsets: host array, num_segments + 1 entries); comp NULL = Less. */
 TRTC_CAPI int n_segmented_sort_by_key(void* ptr_keys, void* ptr_values, const unsigned long long* offsets,
@@ -123,6 +124,37 @@ TRTC_CAPI int n_segmented_sort_by_key(void* ptr_keys, void* ptr_values, const un
 /* Inclusive Plus scan of each segment independently. */
 TRTC_CAPI int n_segmented_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
                                          unsigned long long num_segments);
+/* Keys-only segmented sort; exclusive Plus scan of each segment. */
+TRTC_CAPI int n_segmented_sort(void* ptr_keys, const unsigned long long* offsets, unsigned long long num_segments,
+                               void* ptr_comp);
+TRTC_CAPI int n_segmented_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
+                                         unsigned long long num_segments);
+/* results[s] = reduction of segment s with op (NULL = Plus); segments non-empty. */
+TRTC_CAPI int n_segmented_reduce(void* ptr_vec_in, const unsigned long long* offsets, unsigned long long num_segments,
+                                 void* ptr_results, void* ptr_op);
+/* Tile-schedule autotuning of batched sorts (default on) and its JSON log. */
+TRTC_CAPI void n_set_batch_autotune(int on);
+/* Fixed schedule for every batched sort ("<=2048:256x8 rest:512x16"; NULL or
+ * "" = automatic); -2 if the text is not a valid schedule. */
+TRTC_CAPI int n_set_batch_schedule(const char* schedule);
+TRTC_CAPI unsigned long long n_batch_tuning_report(char* json_out, unsigned long long capacity);
+/* Graph-captured batch: record segmented operations, instantiate once (tune,
+ * allocate, capture one HIP graph), launch many times (one hipGraphLaunch on
+ * the calling thread's stream).  Recorded vectors must stay alive and in place. */
+TRTC_CAPI void* n_batch_create(void);
+TRTC_CAPI void n_batch_destroy(void* batch);
+TRTC_CAPI int n_batch_add_sort(void* batch, void* ptr_keys, const unsigned long long* offsets,
+                               unsigned long long num_segments, void* ptr_comp);
+TRTC_CAPI int n_batch_add_sort_by_key(void* batch, void* ptr_keys, void* ptr_values, const unsigned long long* offsets,
+                                      unsigned long long num_segments, void* ptr_comp);
+TRTC_CAPI int n_batch_add_inclusive_scan(void* batch, void* ptr_vec_in, void* ptr_vec_out,
+                                         const unsigned long long* offsets, unsigned long long num_segments,
+                                         void* ptr_op);
+TRTC_CAPI int n_batch_add_reduce(void* batch, void* ptr_vec_in, const unsigned long long* offsets,
+                                 unsigned long long num_segments, void* ptr_results, void* ptr_op);
+TRTC_CAPI int n_batch_instantiate(void* batch);
+TRTC_CAPI int n_batch_launch(void* batch);
+TRTC_CAPI unsigned long long n_batch_describe(void* batch, char* json_out, unsigned long long capacity);
 /* Stream used by the calling thread (a hipStream_t; NULL = null stream). */
 TRTC_CAPI void n_set_stream(void* stream);
 TRTC_CAPI void* n_get_stream(void);

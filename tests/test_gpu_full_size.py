@@ -1,0 +1,121 @@
+"""GPU parity at the full workload sizes: 2^20 Sort / Inclusive_Scan, 2^28
+Sort_By_Key (uniform, low-entropy and float keys with signed zeros), 2^29
+Inclusive_Scan / Copy_If (x % 3 == 0) / Reduce.  The HIP results are hashed
+(64-bit FNV-1a of the output bytes) and compared with the hashes of the
+oracle's outputs on the same generated inputs (tests/golden/large_hashes.json,
+written by tests/golden/make_large_hashes.py); the input hashes are checked
+first so that a generator change cannot pass silently.  Size-independent
+properties (sortedness, payload is a permutation that reproduces the keys,
+stability of equal keys) are checked as well.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import workloads
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "large_hashes.json")))
+
+
+@pytest.fixture(scope="module")
+def trtc():
+    import thrustrtc_amd
+    return thrustrtc_amd
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return workloads()
+
+
+def h(wl, a):
+    a = np.ascontiguousarray(a)
+    return "%016x" % wl._lib.wl_fnv1a64(a.ctypes.data, a.nbytes)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_sort_u32_2p20(trtc, wl):
+    g = GOLD["sort_u32_2p20_uniform_seed0"]
+    k = wl.u32_uniform(g["n"], 0)
+    assert h(wl, k) == g["keys_in"]
+    d = trtc.device_vector_from_numpy(k)
+    trtc.Sort(d)
+    assert h(wl, d.to_host()) == g["keys_out"]
+
+
+def test_inclusive_scan_i32_2p20(trtc, wl):
+    g = GOLD["inclusive_scan_i32_2p20_seed0"]
+    x = wl.i32_range(g["n"], 0, -100, 100)
+    assert h(wl, x) == g["in"]
+    d = trtc.device_vector_from_numpy(x)
+    trtc.Inclusive_Scan(d, d)
+    assert h(wl, d.to_host()) == g["out"]
+
+
+@pytest.mark.parametrize("case", ["sort_by_key_u32_2p28_uniform_seed2", "sort_by_key_u32_2p28_low_entropy_seed2",
+                                  "sort_by_key_f32_2p28_normal_seed3_zeros4"])
+def test_sort_by_key_2p28(trtc, wl, case):
+    say(f"full-size {case}")
+    g = GOLD[case]
+    n = g["n"]
+    if "uniform" in case:
+        k = wl.u32_uniform(n, 2)
+    elif "low_entropy" in case:
+        k = wl.u32_low_entropy(n, 2)
+    else:
+        k = wl.f32_normal_signed_zeros(n, 3, 4)
+    assert h(wl, k) == g["keys_in"]
+    dk = trtc.device_vector_from_numpy(k)
+    del k
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv)
+    ok, ov = dk.to_host(), dv.to_host()
+    del dk, dv
+    assert h(wl, ok) == g["keys_out"]
+    assert h(wl, ov) == g["values_out"]
+    # properties that do not need the oracle
+    if ok.dtype == np.uint32:
+        assert (ok[1:] >= ok[:-1]).all()
+        eq = ok[1:] == ok[:-1]
+        assert (ov[1:][eq] > ov[:-1][eq]).all()
+
+
+def test_inclusive_scan_i32_2p29(trtc, wl):
+    g = GOLD["inclusive_scan_i32_2p29_seed5"]
+    x = wl.i32_range(g["n"], 5, -1000, 1000)
+    assert h(wl, x) == g["in"]
+    d = trtc.device_vector_from_numpy(x)
+    del x
+    o = trtc.device_vector("int32_t", g["n"])
+    trtc.Inclusive_Scan(d, o)
+    assert h(wl, o.to_host()) == g["out"]
+
+
+def test_copy_if_mod3_i32_2p29(trtc, wl):
+    g = GOLD["copy_if_mod3_i32_2p29_seed5"]
+    x = wl.i32_range(g["n"], 5, -1000, 1000)
+    assert h(wl, x) == g["in"]
+    d = trtc.device_vector_from_numpy(x)
+    del x
+    o = trtc.device_vector("int32_t", g["n"])
+    pred = trtc.Functor({}, ["x"], "        return x % 3 == 0;\n")
+    cnt = trtc.Copy_If(d, o, pred)
+    assert cnt == g["count"]
+    assert h(wl, o.to_host(0, cnt)) == g["out"]
+
+
+def test_reduce_f32_2p29(trtc, wl):
+    g = GOLD["reduce_f32_2p29_seed6"]
+    f = wl.f32_unit(g["n"], 6)
+    assert h(wl, f) == g["in"]
+    d = trtc.device_vector_from_numpy(f)
+    del f
+    r = float(trtc.Reduce(d))
+    assert abs(r - g["sum_fp64"]) <= 1e-5 * abs(g["sum_fp64"])

@@ -88,3 +88,89 @@ def Segmented_Inclusive_Scan(vec_in, vec_out, offsets):
     """Inclusive Plus scan of each segment [offsets[s], offsets[s+1]) independently."""
     arr, ptr, nseg = _offsets(offsets)
     check_i(native.n_segmented_inclusive_scan(vec_in.m_cptr, vec_out.m_cptr, ptr, nseg))
+
+
+def Segmented_Sort(keys, offsets, comp=None):
+    """Keys-only stable sort of each segment independently."""
+    arr, ptr, nseg = _offsets(offsets)
+    check_i(native.n_segmented_sort(keys.m_cptr, ptr, nseg, _p(comp)))
+
+
+def Segmented_Exclusive_Scan(vec_in, vec_out, offsets):
+    """Exclusive Plus scan (init 0) of each segment independently."""
+    arr, ptr, nseg = _offsets(offsets)
+    check_i(native.n_segmented_exclusive_scan(vec_in.m_cptr, vec_out.m_cptr, ptr, nseg))
+
+
+def Segmented_Reduce(vec_in, offsets, results, binary_op=None):
+    """results[s] = reduction of segment s (no init; every segment non-empty)."""
+    arr, ptr, nseg = _offsets(offsets)
+    check_i(native.n_segmented_reduce(vec_in.m_cptr, ptr, nseg, results.m_cptr, _p(binary_op)))
+
+
+def Set_Batch_Autotune(on):
+    native.n_set_batch_autotune(1 if on else 0)
+
+
+def Set_Batch_Schedule(schedule=None):
+    """Fixed tile schedule for every batched sort, e.g. "<=2048:256x8 rest:512x16";
+    None returns to automatic (tuned) schedules."""
+    check_i(native.n_set_batch_schedule(None if schedule is None else schedule.encode()))
+
+
+def _json_out(fn, *args):
+    import json
+    n = fn(*args, None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    fn(*args, buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def Batch_Tuning_Report():
+    """The batched-sort autotuner's runs: signature, candidate times, choice."""
+    return _json_out(native.n_batch_tuning_report)
+
+
+class Batch:
+    """Segmented operations recorded once, captured into one HIP graph by
+    instantiate() and replayed by launch() (one hipGraphLaunch).  The vectors
+    must stay alive while the batch exists (the batch keeps references)."""
+
+    def __init__(self):
+        self.m_cptr = check_cptr(native.n_batch_create())
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "m_cptr", None):
+            native.n_batch_destroy(self.m_cptr)
+            self.m_cptr = None
+
+    def add_sort(self, keys, offsets, comp=None):
+        arr, ptr, nseg = _offsets(offsets)
+        self._keep += [keys, comp]
+        check_i(native.n_batch_add_sort(self.m_cptr, keys.m_cptr, ptr, nseg, _p(comp)))
+
+    def add_sort_by_key(self, keys, values, offsets, comp=None):
+        arr, ptr, nseg = _offsets(offsets)
+        self._keep += [keys, values, comp]
+        check_i(native.n_batch_add_sort_by_key(self.m_cptr, keys.m_cptr, values.m_cptr, ptr, nseg, _p(comp)))
+
+    def add_inclusive_scan(self, vec_in, vec_out, offsets, binary_op=None):
+        arr, ptr, nseg = _offsets(offsets)
+        self._keep += [vec_in, vec_out, binary_op]
+        check_i(native.n_batch_add_inclusive_scan(self.m_cptr, vec_in.m_cptr, vec_out.m_cptr, ptr, nseg,
+                                                  _p(binary_op)))
+
+    def add_reduce(self, vec_in, offsets, results, binary_op=None):
+        arr, ptr, nseg = _offsets(offsets)
+        self._keep += [vec_in, results, binary_op]
+        check_i(native.n_batch_add_reduce(self.m_cptr, vec_in.m_cptr, ptr, nseg, results.m_cptr, _p(binary_op)))
+
+    def instantiate(self):
+        check_i(native.n_batch_instantiate(self.m_cptr))
+
+    def launch(self):
+        check_i(native.n_batch_launch(self.m_cptr))
+
+    def describe(self):
+        return _json_out(native.n_batch_describe, self.m_cptr)

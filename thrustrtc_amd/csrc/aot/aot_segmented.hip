@@ -1,0 +1,70 @@
+// Batched-segment kernels: radix passes and histograms for the tile shapes
+// the batch autotuner chooses among, segmented scans and reductions for the
+// built-in element types.
+#include "aot_common.h"
+
+namespace {
+template <int KB, int VB, int BLOCK, int ITEMS> void reg_seg_pass()
+{
+    std::string p = std::to_string(KB) + ", " + std::to_string(VB) + ", " + std::to_string(BLOCK) + ", " +
+                    std::to_string(ITEMS) + ">";
+    TRTC_REG(("trtc_seg_radix_pass_k<" + p).c_str(), (&trtc_seg_radix_pass_k<KB, VB, BLOCK, ITEMS>));
+}
+template <int KB, int BLOCK, int ITEMS> void reg_seg_hist()
+{
+    std::string p = std::to_string(KB) + ", " + std::to_string(BLOCK) + ", " + std::to_string(ITEMS) + ">";
+    TRTC_REG(("trtc_seg_radix_hist_k<" + p).c_str(), (&trtc_seg_radix_hist_k<KB, BLOCK, ITEMS>));
+}
+template <class T, int ITEMS> void reg_seg_scan(const char* tn, const char* op)
+{
+    std::string name = std::string("trtc_seg_scan_k<") + op + ", " + tn + ", " + std::to_string(ITEMS) + ">";
+    if (std::string(op) == "Plus") TRTC_REG(name.c_str(), (&trtc_seg_scan_k<Plus, T, ITEMS>));
+    else if (std::string(op) == "Maximum") TRTC_REG(name.c_str(), (&trtc_seg_scan_k<Maximum, T, ITEMS>));
+    else TRTC_REG(name.c_str(), (&trtc_seg_scan_k<Minimum, T, ITEMS>));
+}
+template <class T> void reg_seg_reduce(const char* tn)
+{
+    TRTC_REG((std::string("trtc_seg_reduce_k<Plus, ") + tn + ">").c_str(), (&trtc_seg_reduce_k<Plus, T>));
+    TRTC_REG((std::string("trtc_seg_reduce_k<Maximum, ") + tn + ">").c_str(), (&trtc_seg_reduce_k<Maximum, T>));
+    TRTC_REG((std::string("trtc_seg_reduce_k<Minimum, ") + tn + ">").c_str(), (&trtc_seg_reduce_k<Minimum, T>));
+}
+struct RegSeg
+{
+    RegSeg()
+    {
+        // 4-byte keys: keys only and 4-byte values
+        reg_seg_pass<4, 4, 512, 16>();
+        reg_seg_pass<4, 4, 512, 8>();
+        reg_seg_pass<4, 4, 256, 16>();
+        reg_seg_pass<4, 4, 256, 8>();
+        reg_seg_pass<4, 0, 512, 16>();
+        reg_seg_pass<4, 0, 256, 8>();
+        reg_seg_hist<4, 512, 16>();
+        reg_seg_hist<4, 512, 8>();
+        reg_seg_hist<4, 256, 16>();
+        reg_seg_hist<4, 256, 8>();
+        // 8-byte keys with 8-byte or no values
+        reg_seg_pass<8, 8, 512, 8>();
+        reg_seg_pass<8, 8, 256, 8>();
+        reg_seg_pass<8, 0, 512, 8>();
+        reg_seg_pass<8, 0, 256, 8>();
+        reg_seg_hist<8, 512, 8>();
+        reg_seg_hist<8, 256, 8>();
+        for (const char* op : {"Plus", "Maximum", "Minimum"}) {
+            reg_seg_scan<int32_t, 16>("int32_t", op);
+            reg_seg_scan<int32_t, 8>("int32_t", op);
+            reg_seg_scan<uint32_t, 16>("uint32_t", op);
+            reg_seg_scan<uint32_t, 8>("uint32_t", op);
+            reg_seg_scan<float, 16>("float", op);
+            reg_seg_scan<float, 8>("float", op);
+            reg_seg_scan<int64_t, 8>("int64_t", op);
+            reg_seg_scan<double, 8>("double", op);
+        }
+        reg_seg_reduce<int32_t>("int32_t");
+        reg_seg_reduce<uint32_t>("uint32_t");
+        reg_seg_reduce<float>("float");
+        reg_seg_reduce<double>("double");
+        reg_seg_reduce<int64_t>("int64_t");
+    }
+} reg_seg;
+}  // namespace

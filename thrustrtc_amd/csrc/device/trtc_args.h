@@ -76,4 +76,72 @@ struct RadixPassArgs
     int descending;
 };
 
+// ---- batched segments: many independent sequences in one launch --------
+// A launch covers the tiles of a list of segments; tile ids 0..num_tiles-1
+// are handed out by an atomic counter in order, and tile_seg maps each one to
+// its segment.  Segments that span several tiles come first (their tiles use
+// lookback state); single-tile segments follow.
+
+struct SegSortDesc
+{
+    unsigned long long keys, vals;          // device addresses of the segment
+    unsigned long long tmp_keys, tmp_vals;  // ping-pong buffers (== keys/vals for single-tile segments)
+    unsigned long long n;                   // keys in the segment (< 2^30)
+    unsigned first_tile;                    // launch tile id of the segment's tile 0
+    unsigned multi;                         // slot among multi-tile segments, 0xFFFFFFFF if single-tile
+    unsigned long long status_row;          // multi-tile: lookback row of tile 0 (rows of 256 words)
+};
+
+struct SegRadixArgs
+{
+    const SegSortDesc* descs;
+    const unsigned* tile_seg;  // [num_tiles] segment index of each launch tile
+    unsigned num_tiles;
+    unsigned num_multi_tiles;  // tiles 0..num_multi_tiles-1 belong to multi-tile segments
+    unsigned* tile_counter;    // zero on entry (one per pass)
+    unsigned* status;          // [num_multi_tiles][256] lookback words of this pass, zero on entry
+    unsigned* seg_hist;        // [multi segments][KB][256] digit counts, zero on entry
+    unsigned* seg_dstart;      // [multi segments][KB][256] exclusive digit offsets
+    unsigned* seg_done;        // [multi segments] arrival counters, zero on entry
+    int pass;                  // digit index (reads keys when even, tmp when odd)
+    int shift;
+    int kind;
+    int descending;
+};
+
+struct SegScanDesc
+{
+    unsigned long long in, out;  // device addresses
+    unsigned long long n;
+    unsigned first_tile;
+    unsigned pad;
+};
+
+struct SegScanArgs
+{
+    const SegScanDesc* descs;
+    const unsigned* tile_seg;
+    unsigned num_tiles;
+    LookbackArgs lb;
+    int exclusive;
+};
+
+struct SegReduceDesc
+{
+    unsigned long long in;  // device address
+    unsigned long long n;
+    unsigned first_tile;
+    unsigned num_tiles;
+    unsigned long long result;  // device address of one element of T
+};
+
+struct SegReduceArgs
+{
+    const SegReduceDesc* descs;
+    const unsigned* tile_seg;
+    unsigned num_tiles;
+    unsigned long long* partials;  // [num_tiles * words] status words, zero on entry
+    unsigned* seg_done;            // [segments] arrival counters, zero on entry
+};
+
 }  // namespace trtc

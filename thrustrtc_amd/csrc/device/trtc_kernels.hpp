@@ -90,3 +90,32 @@ __global__ void __launch_bounds__(TRTC_RADIX_BLOCK) trtc_radix_pass_k(trtc::Radi
 {
     trtc::radix_pass<TRTC_RADIX_BLOCK, RadixItems<KB, VB>::value, KB, VB, S>(a);
 }
+
+// ---- batched segments (one launch over many independent sequences) ----
+// Tile shapes are template arguments so the batch autotuner can pick among
+// instantiations; every one keeps one thread per digit (BLOCK >= 256).
+template <int KB, int VB, int BLOCK, int ITEMS>
+__global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArgs a)
+{
+    trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB>(a);
+}
+
+template <int KB, int BLOCK, int ITEMS>
+__global__ void __launch_bounds__(BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
+{
+    trtc::seg_radix_hist<BLOCK, BLOCK * ITEMS, KB>(a);
+}
+
+template <class Op, class T, int ITEMS>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_scan_k(Op op, T init, trtc::SegScanArgs a)
+{
+    trtc::seg_scan_tile<TRTC_BLOCK, ITEMS, T>(op, init, a);
+}
+
+// 16384 elements per reduce tile (4 waves x 4096, chunks of 64*ITEMS)
+#define TRTC_SEG_REDUCE_TILE 16384
+template <class Op, class T>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_reduce_k(Op op, trtc::SegReduceArgs a)
+{
+    trtc::seg_reduce_tile<TRTC_BLOCK, trtc::ReduceItems<T>::value, TRTC_SEG_REDUCE_TILE, T>(op, a);
+}

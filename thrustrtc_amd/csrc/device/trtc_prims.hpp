@@ -157,15 +157,16 @@ template <class T, class Op> TRTC_DEV T wave_reverse_reduce(T x, unsigned last, 
     return shfl(x, 0);
 }
 
-// Decoupled lookback run by one full wave for tile `tile` >= 1: returns the
-// inclusive prefix of all tiles before `tile`.  The window of 64 predecessors
+// Decoupled lookback run by one full wave for tile `tile` > `first` (the
+// sequence's first tile): returns the inclusive prefix of the tiles
+// first .. tile-1.  The window of 64 predecessors
 // is read in parallel (lane k reads tile - 1 - k).  A predecessor's
 // inclusive prefix ends the walk; otherwise its aggregate is folded in and
 // the window moves back by 64.  Every tile publishes its aggregate before it
 // looks back and tile ids are handed out in launch order by an atomic counter,
 // so every awaited tile is already resident and the wait ends.
 template <class T, class Op>
-TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op)
+TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op, size_t first = 0)
 {
     unsigned l = lane_id();
     T acc;
@@ -174,8 +175,8 @@ TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op)
     while (true) {
         long long j = top - (long long)l;
         T v;
-        int st = 3;  // 3: before tile 0 (not part of the window)
-        if (j >= 0) {
+        int st = 3;  // 3: before the sequence's first tile (not part of the window)
+        if (j >= (long long)first) {
             // issue both reads before testing either (one latency, not two)
             T vi, va;
             bool oki = try_read<T>(lb.incl, (size_t)j, vi);
@@ -209,6 +210,14 @@ TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op)
     }
     return acc;
 }
+
+// Raw-pointer view of one segment of a batch.
+template <class T> struct PtrView
+{
+    typedef T value_t;
+    T* p;
+    TRTC_DEV T& operator[](size_t i) const { return p[i]; }
+};
 
 // ---------------------------------------------------------------------------
 // Wave-local striped <-> blocked transposition through LDS.

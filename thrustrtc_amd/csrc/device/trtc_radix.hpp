@@ -184,8 +184,23 @@ template <class T> TRTC_DEV T stream_load(const T* p) { return __builtin_nontemp
 //   5. finish the lookback (one thread per digit, LBB predecessors per read),
 //      publish the inclusive prefix,
 //   6. write keys, then values, each digit run to consecutive addresses.
-template <int BLOCK, int ITEMS, int KB, int VB, class S>
-TRTC_DEV void radix_pass(const RadixPassArgs& a)
+// What one workgroup works on in a digit pass: tile `tile` of one key
+// sequence (the whole input of a sort, or one segment of a batch).
+template <class U, class V, class S> struct RadixJob
+{
+    const U* kin;
+    const V* vin;
+    U* kout;
+    V* vout;
+    size_t n;                             // keys in the sequence
+    size_t tile;                          // tile index inside the sequence
+    S* status;                            // lookback rows of the sequence; null for a one-tile sequence
+    const unsigned long long* dstart64;   // [256] digit offsets of the pass, or
+    const unsigned* dstart32;             // [256] (null both: one-tile sequence)
+};
+
+template <int BLOCK, int ITEMS, int KB, int VB, class S, class MakeJob>
+TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, int shift, int kind, int descending)
 {
     typedef typename KeyWord<KB>::U U;
     typedef typename ValWord<VB>::U V;
@@ -208,19 +223,20 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 
     for (int i = threadIdx.x; i < WAVES * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < 256; i += BLOCK) s_hist[i] = 0;
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
     lds_barrier();
 
-    const size_t tile = s_tile;
+    const RadixJob<U, V, S> job = make_job((unsigned)s_tile);
+    const size_t tile = job.tile;
     const unsigned wave = threadIdx.x / 64;
     const unsigned lane = lane_id();
-    const size_t n = a.n;
+    const size_t n = job.n;
     const size_t tile_base = tile * (size_t)TILE;
     const bool full = tile_base + TILE <= n;
     const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
-    const U flip = a.descending ? (U)~(U)0 : (U)0;
-    const U* kin = (const U*)a.keys_in + tile_base;
-    const V* vin = (const V*)a.vals_in + tile_base;
+    const U flip = descending ? (U)~(U)0 : (U)0;
+    const U* kin = job.kin + tile_base;
+    const V* vin = job.vin + tile_base;
     const int wpos0 = (int)wave * 64 * ITEMS;
 
     // 1. loads
@@ -249,19 +265,19 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-        U sk = (full || p < valid) ? sort_key<U>(key[k], a.kind, flip) : (U)~(U)0;
-        dig[k] = (unsigned)(sk >> a.shift) & 255u;
+        U sk = (full || p < valid) ? sort_key<U>(key[k], kind, flip) : (U)~(U)0;
+        dig[k] = (unsigned)(sk >> shift) & 255u;
         __hip_atomic_fetch_add(&s_hist[dig[k]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     lds_barrier();
-    S* st = (S*)a.status;
+    S* st = job.status;
     const bool head = tile == 0;
     unsigned real = 0;
     if (threadIdx.x < 256) {
         const unsigned d = threadIdx.x;
         unsigned c = s_hist[d];
         real = (d == 255 && !full) ? c - (unsigned)(TILE - valid) : c;
-        st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+        if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
     }
 
     // 3. in-wave ranking: peers = lanes with the same digit in this slot;
@@ -356,18 +372,22 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
             }
             st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
         }
-        s_base[d] = (P)(a.digit_start[d] + excl - dstart);
+        // digit offset of the sequence: from the histogram pre-pass, or for a
+        // sequence of one tile, the tile's own digit scan
+        const unsigned long long ds = job.dstart64 ? job.dstart64[d]
+                                    : (job.dstart32 ? (unsigned long long)job.dstart32[d] : (unsigned long long)dstart);
+        s_base[d] = (P)(ds + excl - dstart);
     }
     lds_barrier();
 
     // 6. write each digit run to its global slot
-    U* kout = (U*)a.keys_out;
+    U* kout = job.kout;
     P gpos[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
         unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
         U kb = s_keys[j];
-        unsigned d = (unsigned)(sort_key<U>(kb, a.kind, flip) >> a.shift) & 255u;
+        unsigned d = (unsigned)(sort_key<U>(kb, kind, flip) >> shift) & 255u;
         gpos[r] = s_base[d] + j;
         // gpos < n holds by construction; the check keeps a logic error from
         // ever writing outside the output
@@ -378,11 +398,130 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) s_vals[lpos[k]] = val[k];
         lds_barrier();
-        V* vout = (V*)a.vals_out;
+        V* vout = job.vout;
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
             unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
             if ((full || j < valid) && gpos[r] < (P)n) vout[gpos[r]] = s_vals[j];
+        }
+    }
+}
+
+
+// One digit pass over the whole input.
+template <int BLOCK, int ITEMS, int KB, int VB, class S>
+TRTC_DEV void radix_pass(const RadixPassArgs& a)
+{
+    typedef typename KeyWord<KB>::U U;
+    typedef typename ValWord<VB>::U V;
+    auto make = [&](unsigned t) {
+        RadixJob<U, V, S> j;
+        j.kin = (const U*)a.keys_in;
+        j.vin = (const V*)a.vals_in;
+        j.kout = (U*)a.keys_out;
+        j.vout = (V*)a.vals_out;
+        j.n = a.n;
+        j.tile = t;
+        j.status = (S*)a.status;
+        j.dstart64 = a.digit_start;
+        j.dstart32 = nullptr;
+        return j;
+    };
+    radix_pass_core<BLOCK, ITEMS, KB, VB, S>(a.tile_counter, make, a.shift, a.kind, a.descending);
+}
+
+// One digit pass over a batch of segments.  Even passes read the segment's
+// own buffers and write the ping-pong buffers, odd passes the reverse, so an
+// even number of passes ends in place.  A one-tile segment has no lookback and
+// no pre-pass histogram, and sorts in place (its tile holds every key in
+// registers before it writes).
+template <int BLOCK, int ITEMS, int KB, int VB>
+TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
+{
+    typedef typename KeyWord<KB>::U U;
+    typedef typename ValWord<VB>::U V;
+    constexpr int TILE = BLOCK * ITEMS;
+    auto make = [&](unsigned t) {
+        const SegSortDesc& d = a.descs[a.tile_seg[t]];
+        const bool odd = a.pass & 1;
+        RadixJob<U, V, unsigned> j;
+        j.kin = (const U*)(odd ? d.tmp_keys : d.keys);
+        j.vin = (const V*)(odd ? d.tmp_vals : d.vals);
+        j.kout = (U*)(odd ? d.keys : d.tmp_keys);
+        j.vout = (V*)(odd ? d.vals : d.tmp_vals);
+        j.n = d.n;
+        j.tile = t - d.first_tile;
+        j.dstart64 = nullptr;
+        if (d.n > (unsigned long long)TILE) {
+            j.status = a.status + d.status_row * 256;
+            j.dstart32 = a.seg_dstart + ((size_t)d.multi * KB + a.pass) * 256;
+        } else {
+            j.status = nullptr;
+            j.dstart32 = nullptr;
+        }
+        return j;
+    };
+    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned>(a.tile_counter, make, a.shift, a.kind, a.descending);
+}
+
+// Digit histograms of the multi-tile segments of a batch: one workgroup per
+// tile, all KB digits from one read; the last tile of a segment to finish
+// turns the segment's histograms into exclusive digit offsets.  Counters,
+// histograms and lookback rows are zeroed by the batch driver beforehand.
+template <int BLOCK, int TILE, int KB>
+TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
+{
+    typedef typename KeyWord<KB>::U U;
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ unsigned s_h[WAVES][KB][256];
+    __shared__ bool s_last;
+    for (int i = threadIdx.x; i < WAVES * KB * 256; i += BLOCK) (&s_h[0][0][0])[i] = 0;
+    __syncthreads();
+
+    const unsigned t = blockIdx.x;  // grid = num_multi_tiles
+    const SegSortDesc& d = a.descs[a.tile_seg[t]];
+    const size_t base = (size_t)(t - d.first_tile) * TILE;
+    const size_t cnt = (d.n - base) < (size_t)TILE ? (size_t)(d.n - base) : (size_t)TILE;
+    const U* keys = (const U*)d.keys + base;
+    const unsigned wave = threadIdx.x / 64;
+    const U flip = a.descending ? (U)~(U)0 : (U)0;
+    for (size_t i = threadIdx.x; i < cnt; i += BLOCK) {
+        U k = sort_key<U>(stream_load(keys + i), a.kind, flip);
+#pragma unroll
+        for (int p = 0; p < KB; p++)
+            __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    unsigned* hist = a.seg_hist + (size_t)d.multi * KB * 256;
+    for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
+        unsigned c = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) c += (&s_h[w][0][0])[i];
+        if (c) __hip_atomic_fetch_add(hist + i, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned ntiles = (unsigned)((d.n + TILE - 1) / TILE);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned v = __hip_atomic_fetch_add(a.seg_done + d.multi, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (v == ntiles - 1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (!s_last) return;
+    unsigned* ds = a.seg_dstart + (size_t)d.multi * KB * 256;
+    if (threadIdx.x < 64) {
+        for (int p = 0; p < KB; p++) {
+            unsigned run = 0;
+            for (int c = 0; c < 4; c++) {
+                int dd = c * 64 + (int)threadIdx.x;
+                unsigned v = ld_relaxed(hist + p * 256 + dd);
+                Plus plus;
+                unsigned inc = wave_inclusive_scan(v, plus);
+                ds[p * 256 + dd] = run + inc - v;
+                run += shfl(inc, 63);
+            }
         }
     }
 }

@@ -26,8 +26,12 @@ template <class T> struct MaybeT
     bool has;
 };
 
+// One tile of a sequence: lookback index `tile` (status slot), `first` = the
+// lookback index of the sequence's first tile, elements [base, min(base+TILE,
+// n)) of src/out.
 template <int BLOCK, int ITEMS, class T, class Src, class Out, class Op>
-TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArgs& a)
+TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclusive, const LookbackArgs& lb,
+                           size_t tile, size_t first, size_t tile_base, size_t n)
 {
     constexpr int WAVES = BLOCK / 64;
     constexpr int WELEMS = 64 * ITEMS;
@@ -36,15 +40,9 @@ TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArg
     __shared__ T s_wtot[WAVES];
     __shared__ int s_wvalid[WAVES];
     __shared__ T s_prefix;
-    __shared__ unsigned s_tile;
 
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.lb.tile_counter, 1u);
-    __syncthreads();
-    const size_t tile = s_tile;
     const unsigned wave = threadIdx.x / 64;
     const unsigned lane = lane_id();
-    const size_t n = a.n;
-    const size_t tile_base = tile * TILE;
     const size_t wave_base = tile_base + (size_t)wave * WELEMS;
     const bool full = tile_base + TILE <= n;
     T* lds = s_x[wave];
@@ -113,23 +111,24 @@ TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArg
         tot.has = true;
     }
 
-    // tile prefix P: tile 0 has none (inclusive) or init (exclusive)
+    // tile prefix P: the sequence's first tile has none (inclusive) or init
+    // (exclusive); a sequence of one tile publishes nothing (lb.incl null)
     MaybeT<T> P;
     P.has = false;
-    if (tile == 0) {
-        if (a.exclusive) {
+    if (tile == first) {
+        if (exclusive) {
             P.v = init;
             P.has = true;
-            if (threadIdx.x == 0) publish<T>(a.lb.incl, 0, (T)op(init, tot.v));
-        } else if (threadIdx.x == 0) {
-            publish<T>(a.lb.incl, 0, tot.v);
+            if (threadIdx.x == 0 && lb.incl) publish<T>(lb.incl, tile, (T)op(init, tot.v));
+        } else if (threadIdx.x == 0 && lb.incl) {
+            publish<T>(lb.incl, tile, tot.v);
         }
     } else {
-        if (threadIdx.x == 0) publish<T>(a.lb.agg, tile, tot.v);
+        if (threadIdx.x == 0) publish<T>(lb.agg, tile, tot.v);
         if (wave == 0) {
-            T pre = lookback<T>(a.lb, tile, op);
+            T pre = lookback<T>(lb, tile, op, first);
             if (lane == 0) {
-                publish<T>(a.lb.incl, tile, (T)op(pre, tot.v));
+                publish<T>(lb.incl, tile, (T)op(pre, tot.v));
                 s_prefix = pre;
             }
         }
@@ -150,7 +149,7 @@ TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArg
     }
 
     T y[ITEMS];
-    if (a.exclusive) {
+    if (exclusive) {
         // run.has is always true here (P exists in exclusive mode)
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
@@ -190,6 +189,38 @@ TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArg
             if (wave_base + i < n) out[wave_base + i] = (OutT)lds[WaveTile<ITEMS>::pad(i)];
         }
     }
+}
+
+// One tile of a whole-input scan (tile ids from the atomic counter).
+template <int BLOCK, int ITEMS, class T, class Src, class Out, class Op>
+TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArgs& a)
+{
+    __shared__ unsigned s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.lb.tile_counter, 1u);
+    __syncthreads();
+    const size_t tile = s_tile;
+    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, a.lb, tile, 0, tile * (size_t)BLOCK * ITEMS, a.n);
+}
+
+// One tile of a batch of independent scans: segments of more than one tile
+// use the launch-wide lookback arrays (slots = launch tile ids), each lookback
+// stopping at its segment's first tile.
+template <int BLOCK, int ITEMS, class T, class Op>
+TRTC_DEV void seg_scan_tile(Op& op, const T& init, const SegScanArgs& a)
+{
+    constexpr size_t TILE = (size_t)BLOCK * ITEMS;
+    __shared__ unsigned s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.lb.tile_counter, 1u);
+    __syncthreads();
+    const size_t tile = s_tile;
+    const SegScanDesc& d = a.descs[a.tile_seg[tile]];
+    PtrView<const T> src{(const T*)d.in};
+    PtrView<T> out{(T*)d.out};
+    LookbackArgs lb = a.lb;
+    if (d.n <= TILE) lb.incl = nullptr;
+    const size_t lt = tile - d.first_tile;
+    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, lb, tile, (size_t)d.first_tile, lt * TILE,
+                                  (size_t)d.n);
 }
 
 template <class T> struct ScanItems

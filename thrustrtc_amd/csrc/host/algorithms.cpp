@@ -91,19 +91,7 @@ struct LookbackScratch
     }
 };
 
-bool radix_elem(const std::string& elem, int& key_bytes, int& kind)
-{
-    const ScalarInfo* si = scalar_info(elem);
-    if (!si) return false;
-    switch (si->code) {
-    case TC_I8: case TC_I16: case TC_I32: case TC_I64: kind = 1; break;
-    case TC_U8: case TC_U16: case TC_U32: case TC_U64: case TC_BOOL: kind = 0; break;
-    case TC_F32: case TC_F64: kind = 2; break;
-    default: return false;
-    }
-    key_bytes = (int)si->size;
-    return true;
-}
+bool radix_elem(const std::string& elem, int& key_bytes, int& kind) { return radix_key_info(elem, key_bytes, kind); }
 
 // ---------------------------------------------------------------- radix sort
 // Onesweep LSD radix sort of n keys (key_bytes wide) with optional values

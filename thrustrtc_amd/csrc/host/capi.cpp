@@ -8,6 +8,7 @@
 #include "reduce.h"
 #include "scan.h"
 #include "sort.h"
+#include "batch.h"
 #include "trtc_capi.h"
 #include "trtc_internal.h"
 
@@ -366,6 +367,111 @@ int n_segmented_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsign
                                          num_segments)
                ? 0
                : -1;
+}
+
+int n_segmented_sort(void* ptr_keys, const unsigned long long* offsets, unsigned long long num_segments,
+                     void* ptr_comp)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor less("Less");
+    const Functor& comp = ptr_comp ? *(Functor*)ptr_comp : less;
+    return TRTC_Segmented_Sort(*(DVVectorLike*)ptr_keys, off.data(), num_segments, comp) ? 0 : -1;
+}
+
+int n_segmented_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
+                               unsigned long long num_segments)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    return TRTC_Segmented_Exclusive_Scan(*(DVVectorLike*)ptr_vec_in, *(DVVectorLike*)ptr_vec_out, off.data(),
+                                         num_segments)
+               ? 0
+               : -1;
+}
+
+int n_segmented_reduce(void* ptr_vec_in, const unsigned long long* offsets, unsigned long long num_segments,
+                       void* ptr_results, void* ptr_op)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor plus("Plus");
+    const Functor& op = ptr_op ? *(Functor*)ptr_op : plus;
+    return TRTC_Segmented_Reduce(*(DVVectorLike*)ptr_vec_in, off.data(), num_segments, *(DVVectorLike*)ptr_results,
+                                 op)
+               ? 0
+               : -1;
+}
+
+static unsigned long long copy_json(const std::string& js, char* out, unsigned long long capacity)
+{
+    if (out && capacity) {
+        size_t n = std::min<size_t>(js.size(), capacity - 1);
+        memcpy(out, js.data(), n);
+        out[n] = 0;
+    }
+    return js.size();
+}
+
+void n_set_batch_autotune(int on) { TRTC_Set_Batch_Autotune(on != 0); }
+int n_set_batch_schedule(const char* schedule) { return TRTC_Set_Batch_Schedule(schedule) ? 0 : -2; }
+unsigned long long n_batch_tuning_report(char* json_out, unsigned long long capacity)
+{
+    return copy_json(TRTC_Batch_Tuning_Report(), json_out, capacity);
+}
+
+void* n_batch_create(void) { return new TRTC_Batch; }
+void n_batch_destroy(void* batch) { delete (TRTC_Batch*)batch; }
+
+int n_batch_add_sort(void* batch, void* ptr_keys, const unsigned long long* offsets, unsigned long long num_segments,
+                     void* ptr_comp)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor less("Less");
+    const Functor& comp = ptr_comp ? *(Functor*)ptr_comp : less;
+    return ((TRTC_Batch*)batch)->Add_Sort(*(DVVectorLike*)ptr_keys, off.data(), num_segments, comp) ? 0 : -1;
+}
+
+int n_batch_add_sort_by_key(void* batch, void* ptr_keys, void* ptr_values, const unsigned long long* offsets,
+                            unsigned long long num_segments, void* ptr_comp)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor less("Less");
+    const Functor& comp = ptr_comp ? *(Functor*)ptr_comp : less;
+    return ((TRTC_Batch*)batch)
+                   ->Add_Sort_By_Key(*(DVVectorLike*)ptr_keys, *(DVVectorLike*)ptr_values, off.data(), num_segments,
+                                     comp)
+               ? 0
+               : -1;
+}
+
+int n_batch_add_inclusive_scan(void* batch, void* ptr_vec_in, void* ptr_vec_out, const unsigned long long* offsets,
+                               unsigned long long num_segments, void* ptr_op)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor plus("Plus");
+    const Functor& op = ptr_op ? *(Functor*)ptr_op : plus;
+    return ((TRTC_Batch*)batch)
+                   ->Add_Inclusive_Scan(*(DVVectorLike*)ptr_vec_in, *(DVVectorLike*)ptr_vec_out, off.data(),
+                                        num_segments, op)
+               ? 0
+               : -1;
+}
+
+int n_batch_add_reduce(void* batch, void* ptr_vec_in, const unsigned long long* offsets,
+                       unsigned long long num_segments, void* ptr_results, void* ptr_op)
+{
+    std::vector<size_t> off(offsets, offsets + num_segments + 1);
+    Functor plus("Plus");
+    const Functor& op = ptr_op ? *(Functor*)ptr_op : plus;
+    return ((TRTC_Batch*)batch)
+                   ->Add_Reduce(*(DVVectorLike*)ptr_vec_in, off.data(), num_segments, *(DVVectorLike*)ptr_results, op)
+               ? 0
+               : -1;
+}
+
+int n_batch_instantiate(void* batch) { return ((TRTC_Batch*)batch)->Instantiate() ? 0 : -1; }
+int n_batch_launch(void* batch) { return ((TRTC_Batch*)batch)->Launch() ? 0 : -1; }
+unsigned long long n_batch_describe(void* batch, char* json_out, unsigned long long capacity)
+{
+    return copy_json(((TRTC_Batch*)batch)->Describe(), json_out, capacity);
 }
 
 void n_set_stream(void* stream) { TRTC_Set_Stream(stream); }

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <sstream>
 
@@ -670,12 +671,95 @@ std::string profile_end()
     return js + "}";
 }
 
+namespace {
+thread_local bool t_capturing = false;
+}
+void set_capturing(bool on) { t_capturing = on; }
+bool capturing() { return t_capturing; }
+
+bool launch_graph(hipGraphExec_t g, const char* name, hipStream_t s)
+{
+    Prof& P = prof();
+    ProfRec rec{nullptr, nullptr, nullptr};
+    if (P.on) {
+        std::lock_guard<std::mutex> lk(P.mtx);
+        rec.name = name;
+        rec.a = P.get();
+        rec.b = P.get();
+        hipEventRecord(rec.a, s);
+    }
+    hipError_t e = hipGraphLaunch(g, s);
+    if (rec.name) {
+        std::lock_guard<std::mutex> lk(P.mtx);
+        hipEventRecord(rec.b, s);
+        P.recs.push_back(rec);
+    }
+    if (!check_hip(e, "hipGraphLaunch")) return false;
+    return sync_if_debug(s, "graph execution");
+}
+
+// ---------------------------------------------------------------- pinned staging
+namespace {
+struct Pinned
+{
+    std::mutex mtx;
+    std::multimap<size_t, void*> free_blocks;                    // size -> block
+    std::vector<std::tuple<void*, size_t, hipEvent_t>> pending;  // released, awaiting the stream
+    std::unordered_map<void*, size_t> sizes;
+};
+Pinned& pinned()
+{
+    static Pinned* p = new Pinned();
+    return *p;
+}
+}  // namespace
+
+void* pinned_alloc(size_t bytes)
+{
+    Pinned& P = pinned();
+    std::lock_guard<std::mutex> lk(P.mtx);
+    for (size_t i = 0; i < P.pending.size();) {
+        auto& t = P.pending[i];
+        if (hipEventQuery(std::get<2>(t)) == hipSuccess) {
+            hipEventDestroy(std::get<2>(t));
+            P.free_blocks.emplace(std::get<1>(t), std::get<0>(t));
+            P.pending[i] = P.pending.back();
+            P.pending.pop_back();
+        } else {
+            i++;
+        }
+    }
+    size_t cls = 4096;
+    while (cls < bytes) cls <<= 1;
+    auto it = P.free_blocks.find(cls);
+    if (it != P.free_blocks.end()) {
+        void* p = it->second;
+        P.free_blocks.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    if (!check_hip(hipHostMalloc(&p, cls, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+    P.sizes[p] = cls;
+    return p;
+}
+
+void pinned_release(void* p, hipStream_t s)
+{
+    if (!p) return;
+    Pinned& P = pinned();
+    std::lock_guard<std::mutex> lk(P.mtx);
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return;  // leaks p; never reused
+    hipEventRecord(e, s);
+    P.pending.emplace_back(p, P.sizes[p], e);
+}
+
 bool launch_kernel(const KernelRef& k, dim3 grid, dim3 block, void** args, size_t shmem, hipStream_t s)
 {
     if (grid.x == 0) return true;
     Prof& P = prof();
     ProfRec rec{nullptr, nullptr, nullptr};
-    if (P.on) {
+    if (P.on && !t_capturing) {
         std::lock_guard<std::mutex> lk(P.mtx);
         rec.name = k.name;
         rec.a = P.get();
@@ -694,6 +778,7 @@ bool launch_kernel(const KernelRef& k, dim3 grid, dim3 block, void** args, size_
         P.recs.push_back(rec);
     }
     if (!check_hip(e, "kernel launch")) return false;
+    if (t_capturing) return true;
     return sync_if_debug(s, "kernel execution");
 }
 

@@ -129,10 +129,26 @@ bool sync_if_debug(hipStream_t s, const char* what);
 void profile_begin();
 std::string profile_end();
 
+// While a thread captures a HIP graph, launches skip the timing events and
+// the debug synchronisation (neither may be recorded into a graph).
+void set_capturing(bool on);
+bool capturing();
+// hipGraphLaunch with the same timing hook as kernel launches.
+bool launch_graph(hipGraphExec_t g, const char* name, hipStream_t s);
+
+// Pinned host staging blocks for asynchronous uploads: a released block is
+// reused only after the stream has passed the point of release.
+void* pinned_alloc(size_t bytes);
+void pinned_release(void* p, hipStream_t s);
+
 // Built-in utility kernels.
 bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s);
 bool iota_u32(unsigned* p, size_t n, hipStream_t s);
 bool gather_bytes(const void* src, void* dst, const unsigned* idx, size_t n, size_t elem, hipStream_t s);
+
+// Radix key class of an element type: width in bytes and kind (0 unsigned /
+// bool, 1 signed, 2 floating point); false for other types.
+bool radix_key_info(const std::string& elem, int& key_bytes, int& kind);
 
 // ---------------------------------------------------------------- vectors
 // Raw device pointer of a contiguous vector-like (DVVector, DVVectorAdaptor),

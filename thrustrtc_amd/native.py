@@ -80,6 +80,21 @@ SIGNATURES = {
     "n_sort_by_key": (_i, [_vp, _vp, _vp]),
     "n_segmented_sort_by_key": (_i, [_vp, _vp, ctypes.POINTER(_ull), _ull, _vp]),
     "n_segmented_inclusive_scan": (_i, [_vp, _vp, ctypes.POINTER(_ull), _ull]),
+    "n_segmented_sort": (_i, [_vp, ctypes.POINTER(_ull), _ull, _vp]),
+    "n_segmented_exclusive_scan": (_i, [_vp, _vp, ctypes.POINTER(_ull), _ull]),
+    "n_segmented_reduce": (_i, [_vp, ctypes.POINTER(_ull), _ull, _vp, _vp]),
+    "n_set_batch_autotune": (None, [_i]),
+    "n_set_batch_schedule": (_i, [_s]),
+    "n_batch_tuning_report": (_ull, [ctypes.c_char_p, _ull]),
+    "n_batch_create": (_vp, []),
+    "n_batch_destroy": (None, [_vp]),
+    "n_batch_add_sort": (_i, [_vp, _vp, ctypes.POINTER(_ull), _ull, _vp]),
+    "n_batch_add_sort_by_key": (_i, [_vp, _vp, _vp, ctypes.POINTER(_ull), _ull, _vp]),
+    "n_batch_add_inclusive_scan": (_i, [_vp, _vp, _vp, ctypes.POINTER(_ull), _ull, _vp]),
+    "n_batch_add_reduce": (_i, [_vp, _vp, ctypes.POINTER(_ull), _ull, _vp, _vp]),
+    "n_batch_instantiate": (_i, [_vp]),
+    "n_batch_launch": (_i, [_vp]),
+    "n_batch_describe": (_ull, [_vp, ctypes.c_char_p, _ull]),
     "n_set_stream": (None, [_vp]),
     "n_get_stream": (_vp, []),
     "n_dvvectorlike_data": (_vp, [_vp]),
