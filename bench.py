@@ -310,6 +310,71 @@ def leg_segmented(trtc, native, wl, a, log):
     return res
 
 
+def leg_served(trtc, native, wl, a, log):
+    """Served stream: Poisson arrivals, sizes 2^U(10,24), 60% Sort_By_Key
+    uint32/uint32, 30% Inclusive_Scan int32, 10% Reduce float32 on
+    device-resident inputs; 64 GB admission budget.  The per-call baseline
+    serves one request at a time through the single-call functions; the
+    batched server admits and coalesces continuously on 3 streams.  Offered
+    loads are 0.5x, 1x and 2x the measured per-call saturation rate."""
+    import numpy as np
+    cnt = a.served_requests
+    arena = 1 << 25
+    keys = trtc.device_vector_from_numpy(wl.u32_uniform(arena, 2))
+    vals = trtc.device_vector_from_numpy(np.arange(arena, dtype=np.uint32))
+    xs = trtc.device_vector_from_numpy(wl.i32_range(arena, 5, -1000, 1000))
+    fs = trtc.device_vector_from_numpy(wl.f32_unit(arena, 6))
+    pk, pv = native.n_dvvectorlike_data(keys.m_cptr), native.n_dvvectorlike_data(vals.m_cptr)
+    px, pf = native.n_dvvectorlike_data(xs.m_cptr), native.n_dvvectorlike_data(fs.m_cptr)
+    u = wl.unit_f64(4 * cnt, 8)
+    gaps_unit = -np.log1p(-u[:cnt])                    # exponential, mean 1
+    ns = np.floor(2.0 ** (10.0 + 14.0 * u[cnt:2 * cnt])).astype(np.uint64)
+    ops, ke, ve, pa, pb = [], [], [], [], []
+    for i in range(cnt):
+        n = int(ns[i])
+        off = int((arena - n) * u[3 * cnt + i])
+        m = u[2 * cnt + i]
+        if m < 0.6:
+            ops.append(0); ke.append("uint32_t"); ve.append("uint32_t"); pa.append(pk + 4 * off); pb.append(pv + 4 * off)
+        elif m < 0.9:
+            ops.append(2); ke.append("int32_t"); ve.append(None); pa.append(px + 4 * off); pb.append(None)
+        else:
+            ops.append(3); ke.append("float"); ve.append(None); pa.append(pf + 4 * off); pb.append(None)
+    elems = float(ns.sum())
+    budget = 64 * 10 ** 9
+
+    def replay(batching, rate):
+        srv = trtc.Server(budget, 3 if batching else 1, batching)
+        # warm-up: kernels loaded, pool populated
+        srv.replay(np.zeros(min(cnt, 64)), ops[:64], ke[:64], ve[:64], pa[:64], pb[:64], ns[:64])
+        arr = np.zeros(cnt) if rate is None else np.cumsum(gaps_unit) / rate * 1e6
+        span, lat = srv.replay(arr, ops, ke, ve, pa, pb, ns)
+        st = srv.stats()
+        ok = bool((lat > 0).all())
+        return {"offered_req_per_s": None if rate is None else round(rate, 1),
+                "throughput_req_per_s": round(cnt / span, 1), "throughput_gelem_per_s": round(elems / span / 1e9, 3),
+                "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
+                "mean_us": round(float(lat.mean()), 1), "span_s": round(span, 4), "all_ok": ok,
+                "mean_batch": st["mean_batch"], "peak_held_bytes": st["peak_held_bytes"]}
+
+    per_call_sat = replay(False, None)
+    log(f"served per-call saturation: {per_call_sat}")
+    batched_sat = replay(True, None)
+    log(f"served batched saturation: {batched_sat}")
+    base = per_call_sat["throughput_req_per_s"]
+    loads = {}
+    for f in (0.5, 1.0, 2.0):
+        loads[f"{f}x"] = replay(True, f * base)
+        log(f"served batched at {f}x per-call rate: {loads[f'{f}x']}")
+    per_call_half = replay(False, 0.5 * base)
+    log(f"served per-call at 0.5x: {per_call_half}")
+    return {"requests": cnt, "mix": "60% Sort_By_Key u32/u32, 30% Inclusive_Scan i32, 10% Reduce f32",
+            "sizes": "2^U(10,24)", "budget_bytes": budget, "slots": 3,
+            "per_call_saturation": per_call_sat, "batched_saturation": batched_sat,
+            "throughput_ratio_batched_vs_per_call": round(batched_sat["throughput_req_per_s"] / base, 2),
+            "batched_at_load": loads, "per_call_at_0.5x": per_call_half}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,7 +387,9 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the 2^20 latency leg")
     ap.add_argument("--no-segmented", action="store_true", help="skip the batched-segments leg")
     ap.add_argument("--segments", type=int, default=4096, help="segments of the batched-segments leg")
-    ap.add_argument("--only", choices=["sort", "single_pass", "small", "segmented"], default=None,
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--served-requests", type=int, default=3000, help="requests per served-stream run")
+    ap.add_argument("--only", choices=["sort", "single_pass", "small", "segmented", "served"], default=None,
                     help="run one leg only (the headline fields are then absent)")
     ap.add_argument("--mode", choices=["sort", "copy"], default="sort",
                     help="copy: a plain device copy of a known size (4 B/elem read + 4 B/elem write), "
@@ -354,6 +421,8 @@ def main():
             res = leg_single_pass(trtc, native, wl, a, log)
         elif a.only == "small":
             res = leg_small_calls(trtc, native, wl, log)
+        elif a.only == "served":
+            res = leg_served(trtc, native, wl, a, log)
         else:
             res = leg_segmented(trtc, native, wl, a, log)
         print(json.dumps({"leg": a.only, "result": res}), flush=True)
@@ -459,6 +528,7 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     segmented = None if a.no_segmented else leg_segmented(trtc, native, wl, a, log)
+    served = None if a.no_served else leg_served(trtc, native, wl, a, log)
     single_pass = None if a.no_single_pass else leg_single_pass(trtc, native, wl, a, log)
     small = None if a.no_small else leg_small_calls(trtc, native, wl, log)
     copy_bw = leg_copy(trtc, N, 5, 2)
@@ -490,6 +560,7 @@ def main():
         "cpu_baseline": cpu,
         "single_pass_2p29": single_pass,
         "batched_segments": segmented,
+        "served_stream": served,
         "small_calls_2p20": small,
         "device_copy": copy_bw,
     }

@@ -155,6 +155,24 @@ TRTC_CAPI int n_batch_add_reduce(void* batch, void* ptr_vec_in, const unsigned l
 TRTC_CAPI int n_batch_instantiate(void* batch);
 TRTC_CAPI int n_batch_launch(void* batch);
 TRTC_CAPI unsigned long long n_batch_describe(void* batch, char* json_out, unsigned long long capacity);
+/* Request server (include/server.h): HBM-budget admission and continuous
+ * batching of mixed requests on `slots` streams; batching 0 = one request at
+ * a time through the single-call path.  Ops: 0 Sort_By_Key, 1 Sort,
+ * 2 Inclusive_Scan, 3 Reduce.  Submit returns a ticket or -1. */
+TRTC_CAPI void* n_server_create(unsigned long long hbm_budget, int slots, int batching);
+TRTC_CAPI void n_server_destroy(void* server);
+TRTC_CAPI long long n_server_submit(void* server, int op, const char* key_elem, const char* val_elem, void* in_a,
+                                    void* in_b, unsigned long long n);
+TRTC_CAPI int n_server_wait(void* server, long long ticket);
+TRTC_CAPI int n_server_done(void* server, long long ticket);
+TRTC_CAPI void* n_server_output(void* server, long long ticket, int which);
+TRTC_CAPI double n_server_latency_us(void* server, long long ticket);
+TRTC_CAPI void n_server_release(void* server, long long ticket);
+TRTC_CAPI unsigned long long n_server_stats(void* server, char* json_out, unsigned long long capacity);
+/* Load replay (TRTC_Server_Replay): returns the makespan in seconds. */
+TRTC_CAPI double n_server_replay(void* server, unsigned long long count, const double* arrival_us, const int* ops,
+                                 const char* const* key_elem, const char* const* val_elem, void* const* in_a,
+                                 void* const* in_b, const unsigned long long* n, double* latency_us);
 /* Stream used by the calling thread (a hipStream_t; NULL = null stream). */
 TRTC_CAPI void n_set_stream(void* stream);
 TRTC_CAPI void* n_get_stream(void);

@@ -174,3 +174,70 @@ class Batch:
 
     def describe(self):
         return _json_out(native.n_batch_describe, self.m_cptr)
+
+
+# ---- extension: request server (include/server.h)
+REQ_SORT_BY_KEY, REQ_SORT, REQ_INCLUSIVE_SCAN, REQ_REDUCE = 0, 1, 2, 3
+
+
+class Server:
+    """Request server over device-resident inputs: HBM-budget admission and
+    continuous batching on `slots` streams (batching=False: one request at a
+    time through the single-call path).  Outputs belong to the server until
+    release()."""
+
+    def __init__(self, hbm_budget=64 << 30, slots=3, batching=True):
+        self.m_cptr = check_cptr(native.n_server_create(hbm_budget, slots, 1 if batching else 0))
+
+    def __del__(self):
+        if getattr(self, "m_cptr", None):
+            native.n_server_destroy(self.m_cptr)
+            self.m_cptr = None
+
+    def submit(self, op, key_elem, in_a, n, val_elem=None, in_b=None):
+        """in_a / in_b: device addresses (ints) or DVVector-likes."""
+        def addr(x):
+            if x is None or isinstance(x, int):
+                return x
+            return native.n_dvvectorlike_data(x.m_cptr)
+        t = native.n_server_submit(self.m_cptr, op, key_elem.encode(), None if val_elem is None else val_elem.encode(),
+                                   addr(in_a), addr(in_b), n)
+        if t < 0:
+            raise ValueError("invalid request")
+        return t
+
+    def wait(self, ticket):
+        check_i(native.n_server_wait(self.m_cptr, ticket))
+
+    def done(self, ticket):
+        return native.n_server_done(self.m_cptr, ticket) == 1
+
+    def output(self, ticket, which=0):
+        return native.n_server_output(self.m_cptr, ticket, which)
+
+    def latency_us(self, ticket):
+        return native.n_server_latency_us(self.m_cptr, ticket)
+
+    def release(self, ticket):
+        native.n_server_release(self.m_cptr, ticket)
+
+    def stats(self):
+        return _json_out(native.n_server_stats, self.m_cptr)
+
+    def replay(self, arrival_us, ops, key_elems, val_elems, in_a, in_b, ns):
+        """Submit request i at arrival_us[i] (microseconds from the start);
+        returns (makespan seconds, latency_us array)."""
+        cnt = len(arrival_us)
+        arr = np.ascontiguousarray(arrival_us, dtype=np.float64)
+        opa = np.ascontiguousarray(ops, dtype=np.int32)
+        na = np.ascontiguousarray(ns, dtype=np.uint64)
+        lat = np.zeros(cnt, dtype=np.float64)
+        ke = (ctypes.c_char_p * cnt)(*[k.encode() for k in key_elems])
+        ve = (ctypes.c_char_p * cnt)(*[None if v is None else v.encode() for v in val_elems])
+        pa = (ctypes.c_void_p * cnt)(*in_a)
+        pb = (ctypes.c_void_p * cnt)(*[None if b is None else b for b in in_b])
+        dp = ctypes.POINTER(ctypes.c_double)
+        span = native.n_server_replay(self.m_cptr, cnt, arr.ctypes.data_as(dp), opa.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      ke, ve, pa, pb, na.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)),
+                                      lat.ctypes.data_as(dp))
+        return span, lat

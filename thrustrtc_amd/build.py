@@ -39,7 +39,7 @@ DEVICE_HEADERS = [
     "trtc_kernels.hpp",
 ]
 
-HOST_SOURCES = ["context.cpp", "dvvector.cpp", "algorithms.cpp", "batch.cpp", "segmented.cpp", "capi.cpp"]
+HOST_SOURCES = ["context.cpp", "dvvector.cpp", "algorithms.cpp", "batch.cpp", "segmented.cpp", "server.cpp", "capi.cpp"]
 
 LIB = os.path.join(PKG, "libPyThrustRTC.so")
 WORKLOADS_LIB = os.path.join(PKG, "libtrtc_workloads.so")

@@ -10,10 +10,9 @@ template <int KB, int VB, int BLOCK, int ITEMS> void reg_seg_pass()
                     std::to_string(ITEMS) + ">";
     TRTC_REG(("trtc_seg_radix_pass_k<" + p).c_str(), (&trtc_seg_radix_pass_k<KB, VB, BLOCK, ITEMS>));
 }
-template <int KB, int BLOCK, int ITEMS> void reg_seg_hist()
+template <int KB> void reg_seg_hist()
 {
-    std::string p = std::to_string(KB) + ", " + std::to_string(BLOCK) + ", " + std::to_string(ITEMS) + ">";
-    TRTC_REG(("trtc_seg_radix_hist_k<" + p).c_str(), (&trtc_seg_radix_hist_k<KB, BLOCK, ITEMS>));
+    TRTC_REG(("trtc_seg_radix_hist_k<" + std::to_string(KB) + ">").c_str(), (&trtc_seg_radix_hist_k<KB>));
 }
 template <class T, int ITEMS> void reg_seg_scan(const char* tn, const char* op)
 {
@@ -39,17 +38,13 @@ struct RegSeg
         reg_seg_pass<4, 4, 256, 8>();
         reg_seg_pass<4, 0, 512, 16>();
         reg_seg_pass<4, 0, 256, 8>();
-        reg_seg_hist<4, 512, 16>();
-        reg_seg_hist<4, 512, 8>();
-        reg_seg_hist<4, 256, 16>();
-        reg_seg_hist<4, 256, 8>();
+        reg_seg_hist<4>();
         // 8-byte keys with 8-byte or no values
         reg_seg_pass<8, 8, 512, 8>();
         reg_seg_pass<8, 8, 256, 8>();
         reg_seg_pass<8, 0, 512, 8>();
         reg_seg_pass<8, 0, 256, 8>();
-        reg_seg_hist<8, 512, 8>();
-        reg_seg_hist<8, 256, 8>();
+        reg_seg_hist<8>();
         for (const char* op : {"Plus", "Maximum", "Minimum"}) {
             reg_seg_scan<int32_t, 16>("int32_t", op);
             reg_seg_scan<int32_t, 8>("int32_t", op);

@@ -84,12 +84,15 @@ struct RadixPassArgs
 
 struct SegSortDesc
 {
-    unsigned long long keys, vals;          // device addresses of the segment
+    unsigned long long src_keys, src_vals;  // input of the first pass (== keys/vals for in-place sorts)
+    unsigned long long keys, vals;          // device addresses of the segment (the sorted result)
     unsigned long long tmp_keys, tmp_vals;  // ping-pong buffers (== keys/vals for single-tile segments)
     unsigned long long n;                   // keys in the segment (< 2^30)
     unsigned first_tile;                    // launch tile id of the segment's tile 0
     unsigned multi;                         // slot among multi-tile segments, 0xFFFFFFFF if single-tile
     unsigned long long status_row;          // multi-tile: lookback row of tile 0 (rows of 256 words)
+    unsigned hist_chunks;                   // multi-tile: histogram workgroups of the segment
+    unsigned pad;
 };
 
 struct SegRadixArgs
@@ -103,6 +106,8 @@ struct SegRadixArgs
     unsigned* seg_hist;        // [multi segments][KB][256] digit counts, zero on entry
     unsigned* seg_dstart;      // [multi segments][KB][256] exclusive digit offsets
     unsigned* seg_done;        // [multi segments] arrival counters, zero on entry
+    const unsigned* hist_chunk;  // [num_hist_chunks][2]: segment index, chunk index inside it
+    unsigned num_hist_chunks;
     int pass;                  // digit index (reads keys when even, tmp when odd)
     int shift;
     int kind;

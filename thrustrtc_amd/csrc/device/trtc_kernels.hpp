@@ -100,10 +100,10 @@ __global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArg
     trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB>(a);
 }
 
-template <int KB, int BLOCK, int ITEMS>
-__global__ void __launch_bounds__(BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
+template <int KB>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
 {
-    trtc::seg_radix_hist<BLOCK, BLOCK * ITEMS, KB>(a);
+    trtc::seg_radix_hist<TRTC_BLOCK, KB>(a);
 }
 
 template <class Op, class T, int ITEMS>

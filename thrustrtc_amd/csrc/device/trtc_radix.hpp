@@ -430,9 +430,10 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     radix_pass_core<BLOCK, ITEMS, KB, VB, S>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
-// One digit pass over a batch of segments.  Even passes read the segment's
-// own buffers and write the ping-pong buffers, odd passes the reverse, so an
-// even number of passes ends in place.  A one-tile segment has no lookback and
+// One digit pass over a batch of segments.  Pass 0 reads the segment's
+// source, later even passes its own buffers; even passes write the ping-pong
+// buffers and odd passes the segment's buffers, so an even number of passes
+// ends there.  A one-tile segment has no lookback and
 // no pre-pass histogram, and sorts in place (its tile holds every key in
 // registers before it writes).
 template <int BLOCK, int ITEMS, int KB, int VB>
@@ -445,8 +446,8 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         const SegSortDesc& d = a.descs[a.tile_seg[t]];
         const bool odd = a.pass & 1;
         RadixJob<U, V, unsigned> j;
-        j.kin = (const U*)(odd ? d.tmp_keys : d.keys);
-        j.vin = (const V*)(odd ? d.tmp_vals : d.vals);
+        j.kin = (const U*)(a.pass == 0 ? d.src_keys : (odd ? d.tmp_keys : d.keys));
+        j.vin = (const V*)(a.pass == 0 ? d.src_vals : (odd ? d.tmp_vals : d.vals));
         j.kout = (U*)(odd ? d.keys : d.tmp_keys);
         j.vout = (V*)(odd ? d.vals : d.tmp_vals);
         j.n = d.n;
@@ -465,47 +466,62 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per
-// tile, all KB digits from one read; the last tile of a segment to finish
-// turns the segment's histograms into exclusive digit offsets.  Counters,
-// histograms and lookback rows are zeroed by the batch driver beforehand.
-template <int BLOCK, int TILE, int KB>
+// chunk of SEG_HIST_CHUNK keys (all KB digits from one read, per-wave LDS
+// histograms, one global add per bin and workgroup); the last chunk of a
+// segment to finish turns the segment's histograms into exclusive digit
+// offsets.  Counters and histograms are zeroed by the batch driver.
+#define TRTC_SEG_HIST_CHUNK 131072
+template <int BLOCK, int KB>
 TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
     constexpr int WAVES = BLOCK / 64;
+    constexpr int UNROLL = 8;
     __shared__ unsigned s_h[WAVES][KB][256];
     __shared__ bool s_last;
     for (int i = threadIdx.x; i < WAVES * KB * 256; i += BLOCK) (&s_h[0][0][0])[i] = 0;
     __syncthreads();
 
-    const unsigned t = blockIdx.x;  // grid = num_multi_tiles
-    const SegSortDesc& d = a.descs[a.tile_seg[t]];
-    const size_t base = (size_t)(t - d.first_tile) * TILE;
-    const size_t cnt = (d.n - base) < (size_t)TILE ? (size_t)(d.n - base) : (size_t)TILE;
-    const U* keys = (const U*)d.keys + base;
+    const unsigned c = blockIdx.x;  // grid = num_hist_chunks
+    const unsigned seg = a.hist_chunk[2 * c];
+    const SegSortDesc& d = a.descs[seg];
+    const size_t base = (size_t)a.hist_chunk[2 * c + 1] * TRTC_SEG_HIST_CHUNK;
+    const size_t cnt = (d.n - base) < (size_t)TRTC_SEG_HIST_CHUNK ? (size_t)(d.n - base) : (size_t)TRTC_SEG_HIST_CHUNK;
+    const U* keys = (const U*)d.src_keys + base;
     const unsigned wave = threadIdx.x / 64;
     const U flip = a.descending ? (U)~(U)0 : (U)0;
-    for (size_t i = threadIdx.x; i < cnt; i += BLOCK) {
-        U k = sort_key<U>(stream_load(keys + i), a.kind, flip);
+    for (size_t i0 = 0; i0 < cnt; i0 += (size_t)BLOCK * UNROLL) {
+        U k[UNROLL];
+        bool v[UNROLL];
 #pragma unroll
-        for (int p = 0; p < KB; p++)
-            __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int u = 0; u < UNROLL; u++) {
+            size_t i = i0 + (size_t)u * BLOCK + threadIdx.x;
+            v[u] = i < cnt;
+            if (v[u]) k[u] = stream_load(keys + i);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            if (!v[u]) continue;
+            U sk = sort_key<U>(k[u], a.kind, flip);
+#pragma unroll
+            for (int p = 0; p < KB; p++)
+                __hip_atomic_fetch_add(&s_h[wave][p][(sk >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
     __syncthreads();
     unsigned* hist = a.seg_hist + (size_t)d.multi * KB * 256;
     for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
-        unsigned c = 0;
+        unsigned t = 0;
 #pragma unroll
-        for (int w = 0; w < WAVES; w++) c += (&s_h[w][0][0])[i];
-        if (c) __hip_atomic_fetch_add(hist + i, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int w = 0; w < WAVES; w++) t += (&s_h[w][0][0])[i];
+        if (t) __hip_atomic_fetch_add(hist + i, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned ntiles = (unsigned)((d.n + TILE - 1) / TILE);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         unsigned v = __hip_atomic_fetch_add(a.seg_done + d.multi, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (v == ntiles - 1);
+        s_last = (v == d.hist_chunks - 1);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
@@ -514,12 +530,12 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     if (threadIdx.x < 64) {
         for (int p = 0; p < KB; p++) {
             unsigned run = 0;
-            for (int c = 0; c < 4; c++) {
-                int dd = c * 64 + (int)threadIdx.x;
-                unsigned v = ld_relaxed(hist + p * 256 + dd);
+            for (int q = 0; q < 4; q++) {
+                int dd = q * 64 + (int)threadIdx.x;
+                unsigned x = ld_relaxed(hist + p * 256 + dd);
                 Plus plus;
-                unsigned inc = wave_inclusive_scan(v, plus);
-                ds[p * 256 + dd] = run + inc - v;
+                unsigned inc = wave_inclusive_scan(x, plus);
+                ds[p * 256 + dd] = run + inc - x;
                 run += shfl(inc, 63);
             }
         }

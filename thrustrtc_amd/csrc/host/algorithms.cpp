@@ -95,11 +95,23 @@ bool radix_elem(const std::string& elem, int& key_bytes, int& kind) { return rad
 
 // ---------------------------------------------------------------- radix sort
 // Onesweep LSD radix sort of n keys (key_bytes wide) with optional values
-// (val_bytes in {0,1,2,4,8}); result in place.  Stable.
+// (val_bytes in {0,1,2,4,8}); result in keys/vals.  Stable.  With keys_src
+// (and vals_src) the first pass reads the input from there instead
+// (out-of-place sort, no extra traffic).
 bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool descending, int val_bytes,
-                hipStream_t s)
+                hipStream_t s, const void* keys_src = nullptr, const void* vals_src = nullptr)
 {
+    if (keys_src && n == 1) {
+        if (!check_hip(hipMemcpyAsync(keys, keys_src, key_bytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+            return false;
+        return !val_bytes ||
+               check_hip(hipMemcpyAsync(vals, vals_src, val_bytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+    }
     if (n <= 1) return true;
+    if (!keys_src) {
+        keys_src = keys;
+        vals_src = vals;
+    }
     const int KB = key_bytes;
     // tile shape of trtc_radix_pass_k (RadixItems in the device header)
     const unsigned kRadixBlock = 512;
@@ -129,7 +141,7 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     std::string hname = "trtc_radix_hist_k<" + std::to_string(KB) + ">";
     if (!get_template_kernel(hname, "", hk)) return false;
     trtc::RadixHistArgs ha;
-    ha.keys = keys;
+    ha.keys = keys_src;
     ha.n = n;
     ha.kind = kind;
     ha.descending = descending ? 1 : 0;
@@ -153,9 +165,9 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     void* kbuf[2] = {keys, ktmp.p};
     void* vbuf[2] = {vals, vtmp.p};
     for (int p = 0; p < KB; p++) {
-        pa.keys_in = kbuf[p & 1];
+        pa.keys_in = p == 0 ? keys_src : kbuf[p & 1];
         pa.keys_out = kbuf[(p + 1) & 1];
-        pa.vals_in = vbuf[p & 1];
+        pa.vals_in = p == 0 ? vals_src : vbuf[p & 1];
         pa.vals_out = vbuf[(p + 1) & 1];
         pa.n = n;
         pa.digit_start = (const unsigned long long*)(sc + off_dstart) + p * 256;
@@ -381,6 +393,14 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
 }
 
 }  // namespace
+
+namespace trtc_host {
+bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,
+                     int kind, bool descending, int val_bytes, hipStream_t s)
+{
+    return radix_sort(keys, vals, n, key_bytes, kind, descending, val_bytes, s, keys_src, vals_src);
+}
+}  // namespace trtc_host
 
 // ================================================================ public API
 bool TRTC_Sort(DVVectorLike& vec, const Functor& comp) { return sort_dispatch(vec, nullptr, comp); }

@@ -14,6 +14,7 @@ namespace {
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 const size_t kNoLimit = ~(size_t)0;
+const size_t kHistChunk = 131072;  // TRTC_SEG_HIST_CHUNK in the device header
 }  // namespace
 
 bool upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
@@ -85,7 +86,7 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         plan.cls[c].tile = (size_t)sched[c].block * sched[c].items;
     }
     for (const SortSeg& g : segs) {
-        if (g.n < 2) continue;
+        if (g.n == 0 || (g.n == 1 && !g.src_keys)) continue;  // nothing to move
         if (g.n >= (1ull << 30)) {
             TRTC_ERR("batched sort: segment of %zu keys (limit 2^30 - 1); sort it with TRTC_Sort_By_Key", g.n);
             return false;
@@ -112,6 +113,7 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
             if (nt > 1) {
                 C.multi_tiles += (unsigned)nt;
                 C.multi_segs++;
+                C.hist_chunks += (unsigned)ceil_div(C.segs[i].n, kHistChunk);
                 C.tmp_off[i] = plan.tmp_elems;
                 plan.tmp_elems += C.segs[i].n;
             }
@@ -120,6 +122,8 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         off += align256(C.segs.size() * sizeof(trtc::SegSortDesc));
         C.off_tile_seg = off;
         off += align256((size_t)C.num_tiles * 4);
+        C.off_hist_chunk = off;
+        off += align256((size_t)C.hist_chunks * 8);
     }
     plan.table_bytes = off;
     for (auto& C : plan.cls) {
@@ -147,7 +151,8 @@ void fill_sort_tables(const SortPlan& plan, char* scratch, char* tmp_keys, char*
     for (const auto& C : plan.cls) {
         trtc::SegSortDesc* D = (trtc::SegSortDesc*)(table + C.off_descs);
         unsigned* TS = (unsigned*)(table + C.off_tile_seg);
-        unsigned t = 0, m = 0;
+        unsigned* HC = (unsigned*)(table + C.off_hist_chunk);
+        unsigned t = 0, m = 0, hc = 0;
         unsigned long long row = 0;
         for (size_t i = 0; i < C.segs.size(); i++) {
             const SortSeg& g = C.segs[i];
@@ -155,6 +160,8 @@ void fill_sort_tables(const SortPlan& plan, char* scratch, char* tmp_keys, char*
             trtc::SegSortDesc& d = D[i];
             d.keys = (unsigned long long)g.keys;
             d.vals = (unsigned long long)g.vals;
+            d.src_keys = (unsigned long long)(g.src_keys ? g.src_keys : g.keys);
+            d.src_vals = (unsigned long long)(g.src_keys ? g.src_vals : g.vals);
             d.n = g.n;
             d.first_tile = t;
             if (nt > 1) {
@@ -163,23 +170,27 @@ void fill_sort_tables(const SortPlan& plan, char* scratch, char* tmp_keys, char*
                 d.multi = m++;
                 d.status_row = row;
                 row += nt;
+                d.hist_chunks = (unsigned)ceil_div(g.n, kHistChunk);
+                for (unsigned c = 0; c < d.hist_chunks; c++) {
+                    HC[2 * hc] = (unsigned)i;
+                    HC[2 * hc + 1] = c;
+                    hc++;
+                }
             } else {
                 d.tmp_keys = d.keys;
                 d.tmp_vals = d.vals;
                 d.multi = 0xFFFFFFFFu;
                 d.status_row = 0;
+                d.hist_chunks = 0;
             }
+            d.pad = 0;
             for (size_t k = 0; k < nt; k++) TS[t++] = (unsigned)i;
         }
     }
 }
 
 namespace {
-std::string seg_hist_name(const SortPlan& plan, const SortPlan::Cls& C)
-{
-    return "trtc_seg_radix_hist_k<" + std::to_string(plan.KB) + ", " + std::to_string(C.block) + ", " +
-           std::to_string(C.items) + ">";
-}
+std::string seg_hist_name(const SortPlan& plan) { return "trtc_seg_radix_hist_k<" + std::to_string(plan.KB) + ">"; }
 std::string seg_pass_name(const SortPlan& plan, const SortPlan::Cls& C)
 {
     return "trtc_seg_radix_pass_k<" + std::to_string(plan.KB) + ", " + std::to_string(plan.VB) + ", " +
@@ -197,7 +208,7 @@ bool prepare_sort(const SortPlan& plan)
     KernelRef k;
     for (const auto& C : plan.cls) {
         if (!C.num_tiles) continue;
-        if (C.multi_tiles && !get_template_kernel(seg_hist_name(plan, C), "", k)) return false;
+        if (C.multi_tiles && !get_template_kernel(seg_hist_name(plan), "", k)) return false;
         if (!get_template_kernel(seg_pass_name(plan, C), "", k)) return false;
     }
     return true;
@@ -230,15 +241,17 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
         a.seg_hist = (unsigned*)(scratch + C.off_hist);
         a.seg_dstart = (unsigned*)(scratch + C.off_dstart);
         a.seg_done = (unsigned*)(scratch + C.off_done);
+        a.hist_chunk = (const unsigned*)(scratch + C.off_hist_chunk);
+        a.num_hist_chunks = C.hist_chunks;
         a.pass = 0;
         a.shift = 0;
         a.kind = plan.kind;
         a.descending = plan.desc ? 1 : 0;
         if (C.multi_tiles) {
             KernelRef hk;
-            if (!get_template_kernel(seg_hist_name(plan, C), "", hk)) return false;
+            if (!get_template_kernel(seg_hist_name(plan), "", hk)) return false;
             void* args[] = {&a};
-            if (!launch_kernel(hk, dim3(C.multi_tiles), dim3(C.block), args, 0, s)) return false;
+            if (!launch_kernel(hk, dim3(C.hist_chunks), dim3(256), args, 0, s)) return false;
         }
         KernelRef pk;
         if (!get_template_kernel(seg_pass_name(plan, C), "", pk)) return false;
@@ -276,6 +289,16 @@ bool copy_back_odd(const SortPlan& plan, char* tmp_keys, char* tmp_vals, hipStre
 }
 }  // namespace
 
+bool launch_sort_plan(const SortPlan& plan, char* scratch, char* tmp_keys, char* tmp_vals, hipStream_t s)
+{
+    if (!plan.keys_total) return true;
+    std::vector<char> table(plan.table_bytes);
+    fill_sort_tables(plan, scratch, tmp_keys, tmp_vals, table.data());
+    if (!upload_async(scratch, table.data(), table.size(), s)) return false;
+    if (!enqueue_sort(plan, scratch, s)) return false;
+    return copy_back_odd(plan, tmp_keys, tmp_vals, s);
+}
+
 bool run_sort_batch(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool descending,
                     const SortSchedule& sched, hipStream_t s)
 {
@@ -285,11 +308,7 @@ bool run_sort_batch(const std::vector<SortSeg>& segs, int KB, int VB, int kind, 
     PoolBuf scratch(plan.scratch_bytes, s);
     PoolBuf tk(plan.tmp_elems * KB, s), tv(plan.tmp_elems * VB, s);
     if (!scratch.p || (plan.tmp_elems && !tk.p) || (plan.tmp_elems && VB && !tv.p)) return false;
-    std::vector<char> table(plan.table_bytes);
-    fill_sort_tables(plan, (char*)scratch.p, (char*)tk.p, (char*)tv.p, table.data());
-    if (!upload_async(scratch.p, table.data(), table.size(), s)) return false;
-    if (!enqueue_sort(plan, (char*)scratch.p, s)) return false;
-    return copy_back_odd(plan, (char*)tk.p, (char*)tv.p, s);
+    return launch_sort_plan(plan, (char*)scratch.p, (char*)tk.p, (char*)tv.p, s);
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -347,16 +366,18 @@ bool fill_shadow(const std::vector<SortSeg>& segs, const std::vector<SortSeg>& s
     size_t i = 0;
     while (i < segs.size()) {
         size_t j = i + 1, n = segs[i].n;
-        while (j < segs.size() && segs[j].keys == segs[j - 1].keys + segs[j - 1].n * KB &&
+        while (j < segs.size() && !segs[j].src_keys && !segs[i].src_keys &&
+               segs[j].keys == segs[j - 1].keys + segs[j - 1].n * KB &&
                (!VB || segs[j].vals == segs[j - 1].vals + segs[j - 1].n * VB)) {
             n += segs[j].n;
             j++;
         }
         if (n) {
-            if (!check_hip(hipMemcpyAsync(shadow[i].keys, segs[i].keys, n * KB, hipMemcpyDeviceToDevice, s),
-                           "hipMemcpyAsync"))
+            const char* sk = segs[i].src_keys ? segs[i].src_keys : segs[i].keys;
+            const char* sv = segs[i].src_keys ? segs[i].src_vals : segs[i].vals;
+            if (!check_hip(hipMemcpyAsync(shadow[i].keys, sk, n * KB, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
                 return false;
-            if (VB && !check_hip(hipMemcpyAsync(shadow[i].vals, segs[i].vals, n * VB, hipMemcpyDeviceToDevice, s),
+            if (VB && !check_hip(hipMemcpyAsync(shadow[i].vals, sv, n * VB, hipMemcpyDeviceToDevice, s),
                                  "hipMemcpyAsync"))
                 return false;
         }
@@ -460,6 +481,7 @@ SortSchedule schedule_for(const std::vector<SortSeg>& segs, int KB, int VB, int 
     std::vector<SortSeg> shadow(live.size());
     size_t off = 0;
     for (size_t i = 0; i < live.size(); i++) {
+        shadow[i] = SortSeg();
         shadow[i].keys = (char*)sk.p + off * KB;
         shadow[i].vals = VB ? (char*)sv.p + off * VB : nullptr;
         shadow[i].n = live[i].n;
@@ -590,10 +612,16 @@ bool run_scan_batch(const std::vector<ScanSeg>& segs, const std::string& T, cons
     if (!plan.num_tiles) return true;
     PoolBuf scratch(plan.scratch_bytes, s);
     if (!scratch.p) return false;
+    return launch_scan_plan(plan, (char*)scratch.p, s);
+}
+
+bool launch_scan_plan(const ScanPlan& plan, char* scratch, hipStream_t s)
+{
+    if (!plan.num_tiles) return true;
     std::vector<char> table(plan.table_bytes);
-    fill_scan_tables(plan, (char*)scratch.p, table.data());
-    if (!upload_async(scratch.p, table.data(), table.size(), s)) return false;
-    return enqueue_scan(plan, (char*)scratch.p, s);
+    fill_scan_tables(plan, scratch, table.data());
+    if (!upload_async(scratch, table.data(), table.size(), s)) return false;
+    return enqueue_scan(plan, scratch, s);
 }
 
 // ================================================================ reduce
@@ -681,10 +709,16 @@ bool run_reduce_batch(const std::vector<ReduceSeg>& segs, const std::string& T, 
     if (!plan.num_tiles) return true;
     PoolBuf scratch(plan.scratch_bytes, s);
     if (!scratch.p) return false;
+    return launch_reduce_plan(plan, (char*)scratch.p, s);
+}
+
+bool launch_reduce_plan(const ReducePlan& plan, char* scratch, hipStream_t s)
+{
+    if (!plan.num_tiles) return true;
     std::vector<char> table(plan.table_bytes);
-    fill_reduce_tables(plan, (char*)scratch.p, table.data());
-    if (!upload_async(scratch.p, table.data(), table.size(), s)) return false;
-    return enqueue_reduce(plan, (char*)scratch.p, s);
+    fill_reduce_tables(plan, scratch, table.data());
+    if (!upload_async(scratch, table.data(), table.size(), s)) return false;
+    return enqueue_reduce(plan, scratch, s);
 }
 
 }  // namespace trtc_host

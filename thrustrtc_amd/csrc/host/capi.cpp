@@ -9,6 +9,7 @@
 #include "scan.h"
 #include "sort.h"
 #include "batch.h"
+#include "server.h"
 #include "trtc_capi.h"
 #include "trtc_internal.h"
 
@@ -472,6 +473,38 @@ int n_batch_launch(void* batch) { return ((TRTC_Batch*)batch)->Launch() ? 0 : -1
 unsigned long long n_batch_describe(void* batch, char* json_out, unsigned long long capacity)
 {
     return copy_json(((TRTC_Batch*)batch)->Describe(), json_out, capacity);
+}
+
+void* n_server_create(unsigned long long hbm_budget, int slots, int batching)
+{
+    if (!TRTC_Try_Init()) return nullptr;
+    return new TRTC_Server((size_t)hbm_budget, slots, batching != 0);
+}
+void n_server_destroy(void* server) { delete (TRTC_Server*)server; }
+long long n_server_submit(void* server, int op, const char* key_elem, const char* val_elem, void* in_a, void* in_b,
+                          unsigned long long n)
+{
+    return ((TRTC_Server*)server)->Submit(op, key_elem, val_elem, in_a, in_b, (size_t)n);
+}
+int n_server_wait(void* server, long long ticket) { return ((TRTC_Server*)server)->Wait(ticket) ? 0 : -1; }
+int n_server_done(void* server, long long ticket) { return ((TRTC_Server*)server)->Done(ticket) ? 1 : 0; }
+void* n_server_output(void* server, long long ticket, int which)
+{
+    return ((TRTC_Server*)server)->Output(ticket, which);
+}
+double n_server_latency_us(void* server, long long ticket) { return ((TRTC_Server*)server)->Latency_Us(ticket); }
+void n_server_release(void* server, long long ticket) { ((TRTC_Server*)server)->Release(ticket); }
+unsigned long long n_server_stats(void* server, char* json_out, unsigned long long capacity)
+{
+    return copy_json(((TRTC_Server*)server)->Stats(), json_out, capacity);
+}
+double n_server_replay(void* server, unsigned long long count, const double* arrival_us, const int* ops,
+                       const char* const* key_elem, const char* const* val_elem, void* const* in_a,
+                       void* const* in_b, const unsigned long long* n, double* latency_us)
+{
+    std::vector<size_t> nn(n, n + count);
+    return TRTC_Server_Replay(*(TRTC_Server*)server, (size_t)count, arrival_us, ops, key_elem, val_elem,
+                              (const void* const*)in_a, (const void* const*)in_b, nn.data(), latency_us);
 }
 
 void n_set_stream(void* stream) { TRTC_Set_Stream(stream); }

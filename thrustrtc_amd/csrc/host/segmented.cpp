@@ -62,7 +62,11 @@ bool sort_spec(DVVectorLike& keys, DVVectorLike* values, const size_t* offsets, 
     if (!check_offsets(offsets, num_segments, limit, who)) return false;
     for (size_t s = 0; s < num_segments; s++) {
         size_t b = offsets[s], n = offsets[s + 1] - offsets[s];
-        sp.segs.push_back({kp + b * sp.KB, vp ? vp + b * sp.VB : nullptr, n});
+        SortSeg g;
+        g.keys = kp + b * sp.KB;
+        g.vals = vp ? vp + b * sp.VB : nullptr;
+        g.n = n;
+        sp.segs.push_back(g);
     }
     return true;
 }

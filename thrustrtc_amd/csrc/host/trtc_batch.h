@@ -16,6 +16,8 @@ struct SortSeg
     char* keys;
     char* vals;  // null for keys-only sorts
     size_t n;
+    const char* src_keys = nullptr;  // read by the first pass instead of keys (out-of-place sort)
+    const char* src_vals = nullptr;
 };
 
 // Segments of up to max_n keys run in a launch with tiles of block*items keys.
@@ -39,8 +41,8 @@ struct SortPlan
         size_t tile = 8192;
         std::vector<SortSeg> segs;  // multi-tile segments first (largest first), then one-tile ones
         std::vector<size_t> tmp_off;  // element offset into the ping-pong buffers (multi-tile only)
-        unsigned num_tiles = 0, multi_tiles = 0, multi_segs = 0;
-        size_t off_descs = 0, off_tile_seg = 0;      // in the uploaded table block
+        unsigned num_tiles = 0, multi_tiles = 0, multi_segs = 0, hist_chunks = 0;
+        size_t off_descs = 0, off_tile_seg = 0, off_hist_chunk = 0;  // in the uploaded table block
         size_t off_zero = 0, zero_bytes = 0;         // in scratch: cleared before the launches
         size_t off_cnt = 0, off_done = 0, off_hist = 0, off_status = 0, status_pass_words = 0;
         size_t off_dstart = 0;
@@ -52,7 +54,8 @@ struct SortPlan
     size_t keys_total = 0;
 };
 
-// Sizes and layouts only (no device work).  Segments with n < 2 are dropped.
+// Sizes and layouts only (no device work).  Segments with nothing to move
+// (empty, or one key sorted in place) are dropped.
 // Every segment must hold fewer than 2^30 keys.
 bool plan_sort(const std::vector<SortSeg>& segs, int key_bytes, int val_bytes, int kind, bool descending,
                const SortSchedule& sched, SortPlan& plan);
@@ -65,6 +68,10 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s);
 // Resolves (compiling if needed) every kernel the plan launches, so that a
 // later enqueue under graph capture does no compilation or module loading.
 bool prepare_sort(const SortPlan& plan);
+// Tables + upload + enqueue (+ copy-back for odd digit counts) with caller
+// memory: scratch of plan.scratch_bytes, ping-pong buffers of
+// plan.tmp_elems keys / values.
+bool launch_sort_plan(const SortPlan& plan, char* scratch, char* tmp_keys, char* tmp_vals, hipStream_t s);
 // One-shot: plan, allocate from the pool, upload, enqueue.
 bool run_sort_batch(const std::vector<SortSeg>& segs, int key_bytes, int val_bytes, int kind, bool descending,
                     const SortSchedule& sched, hipStream_t s);
@@ -111,6 +118,7 @@ bool plan_scan(const std::vector<ScanSeg>& segs, const std::string& T, const Fun
 void fill_scan_tables(const ScanPlan& plan, char* scratch, char* table);
 bool enqueue_scan(const ScanPlan& plan, char* scratch, hipStream_t s);
 bool prepare_scan(const ScanPlan& plan);
+bool launch_scan_plan(const ScanPlan& plan, char* scratch, hipStream_t s);
 bool run_scan_batch(const std::vector<ScanSeg>& segs, const std::string& T, const Functor& op, const ViewBuf& init,
                     bool exclusive, int items, hipStream_t s);
 int default_scan_items(size_t tsz);
@@ -138,6 +146,7 @@ bool plan_reduce(const std::vector<ReduceSeg>& segs, const std::string& T, const
 void fill_reduce_tables(const ReducePlan& plan, char* scratch, char* table);
 bool enqueue_reduce(const ReducePlan& plan, char* scratch, hipStream_t s);
 bool prepare_reduce(const ReducePlan& plan);
+bool launch_reduce_plan(const ReducePlan& plan, char* scratch, hipStream_t s);
 bool run_reduce_batch(const std::vector<ReduceSeg>& segs, const std::string& T, const Functor& op, hipStream_t s);
 
 // Upload `bytes` of host data to device `dst` on stream s through a pinned

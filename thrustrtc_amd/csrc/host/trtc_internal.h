@@ -150,6 +150,11 @@ bool gather_bytes(const void* src, void* dst, const unsigned* idx, size_t n, siz
 // bool, 1 signed, 2 floating point); false for other types.
 bool radix_key_info(const std::string& elem, int& key_bytes, int& kind);
 
+// Single-call onesweep radix sort (the TRTC_Sort_By_Key path) reading its
+// input from keys_src/vals_src and leaving the result in keys/vals.
+bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,
+                     int kind, bool descending, int val_bytes, hipStream_t s);
+
 // ---------------------------------------------------------------- vectors
 // Raw device pointer of a contiguous vector-like (DVVector, DVVectorAdaptor),
 // nullptr otherwise.

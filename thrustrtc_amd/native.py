@@ -95,6 +95,17 @@ SIGNATURES = {
     "n_batch_instantiate": (_i, [_vp]),
     "n_batch_launch": (_i, [_vp]),
     "n_batch_describe": (_ull, [_vp, ctypes.c_char_p, _ull]),
+    "n_server_create": (_vp, [_ull, _i, _i]),
+    "n_server_destroy": (None, [_vp]),
+    "n_server_submit": (ctypes.c_longlong, [_vp, _i, _s, _s, _vp, _vp, _ull]),
+    "n_server_wait": (_i, [_vp, ctypes.c_longlong]),
+    "n_server_done": (_i, [_vp, ctypes.c_longlong]),
+    "n_server_output": (_vp, [_vp, ctypes.c_longlong, _i]),
+    "n_server_latency_us": (_d, [_vp, ctypes.c_longlong]),
+    "n_server_release": (None, [_vp, ctypes.c_longlong]),
+    "n_server_stats": (_ull, [_vp, ctypes.c_char_p, _ull]),
+    "n_server_replay": (_d, [_vp, _ull, ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_s), ctypes.POINTER(_s),
+                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_ull), ctypes.POINTER(_d)]),
     "n_set_stream": (None, [_vp]),
     "n_get_stream": (_vp, []),
     "n_dvvectorlike_data": (_vp, [_vp]),
