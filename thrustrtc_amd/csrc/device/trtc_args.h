@@ -108,6 +108,7 @@ struct SegRadixArgs
     unsigned* seg_done;        // [multi segments] arrival counters, zero on entry
     const unsigned* hist_chunk;  // [num_hist_chunks][2]: segment index, chunk index inside it
     unsigned num_hist_chunks;
+    unsigned hist_chunk_keys;    // keys per histogram chunk
     int pass;                  // digit index (reads keys when even, tmp when odd)
     int shift;
     int kind;

@@ -466,11 +466,10 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per
-// chunk of SEG_HIST_CHUNK keys (all KB digits from one read, per-wave LDS
+// chunk of hist_chunk_keys keys (all KB digits from one read, per-wave LDS
 // histograms, one global add per bin and workgroup); the last chunk of a
 // segment to finish turns the segment's histograms into exclusive digit
 // offsets.  Counters and histograms are zeroed by the batch driver.
-#define TRTC_SEG_HIST_CHUNK 131072
 template <int BLOCK, int KB>
 TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
 {
@@ -485,8 +484,9 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     const unsigned c = blockIdx.x;  // grid = num_hist_chunks
     const unsigned seg = a.hist_chunk[2 * c];
     const SegSortDesc& d = a.descs[seg];
-    const size_t base = (size_t)a.hist_chunk[2 * c + 1] * TRTC_SEG_HIST_CHUNK;
-    const size_t cnt = (d.n - base) < (size_t)TRTC_SEG_HIST_CHUNK ? (size_t)(d.n - base) : (size_t)TRTC_SEG_HIST_CHUNK;
+    const size_t ck = a.hist_chunk_keys;
+    const size_t base = (size_t)a.hist_chunk[2 * c + 1] * ck;
+    const size_t cnt = (d.n - base) < ck ? (size_t)(d.n - base) : ck;
     const U* keys = (const U*)d.src_keys + base;
     const unsigned wave = threadIdx.x / 64;
     const U flip = a.descending ? (U)~(U)0 : (U)0;

@@ -14,7 +14,6 @@ namespace {
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 const size_t kNoLimit = ~(size_t)0;
-const size_t kHistChunk = 131072;  // TRTC_SEG_HIST_CHUNK in the device header
 }  // namespace
 
 bool upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
@@ -96,6 +95,18 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         plan.cls[c].segs.push_back(g);
         plan.keys_total += g.n;
     }
+    // histogram chunk: 8K..128K keys, small enough that the batch's multi-tile
+    // keys give every CU a few workgroups
+    {
+        size_t multi_keys = 0;
+        for (auto& C : plan.cls)
+            for (auto& g : C.segs)
+                if (g.n > C.tile) multi_keys += g.n;
+        size_t want = multi_keys / ((size_t)std::max(1, context().num_cus) * 4);
+        size_t ck = 8192;
+        while (ck < want && ck < 131072) ck <<= 1;
+        plan.hist_chunk = ck;
+    }
     size_t off = 0;
     for (auto& C : plan.cls) {
         // multi-tile segments first, largest first (they start earliest and
@@ -113,7 +124,7 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
             if (nt > 1) {
                 C.multi_tiles += (unsigned)nt;
                 C.multi_segs++;
-                C.hist_chunks += (unsigned)ceil_div(C.segs[i].n, kHistChunk);
+                C.hist_chunks += (unsigned)ceil_div(C.segs[i].n, plan.hist_chunk);
                 C.tmp_off[i] = plan.tmp_elems;
                 plan.tmp_elems += C.segs[i].n;
             }
@@ -170,7 +181,7 @@ void fill_sort_tables(const SortPlan& plan, char* scratch, char* tmp_keys, char*
                 d.multi = m++;
                 d.status_row = row;
                 row += nt;
-                d.hist_chunks = (unsigned)ceil_div(g.n, kHistChunk);
+                d.hist_chunks = (unsigned)ceil_div(g.n, plan.hist_chunk);
                 for (unsigned c = 0; c < d.hist_chunks; c++) {
                     HC[2 * hc] = (unsigned)i;
                     HC[2 * hc + 1] = c;
@@ -243,6 +254,7 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
         a.seg_done = (unsigned*)(scratch + C.off_done);
         a.hist_chunk = (const unsigned*)(scratch + C.off_hist_chunk);
         a.num_hist_chunks = C.hist_chunks;
+        a.hist_chunk_keys = (unsigned)plan.hist_chunk;
         a.pass = 0;
         a.shift = 0;
         a.kind = plan.kind;

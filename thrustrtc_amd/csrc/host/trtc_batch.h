@@ -48,6 +48,7 @@ struct SortPlan
         size_t off_dstart = 0;
     };
     std::vector<Cls> cls;
+    size_t hist_chunk = 131072;  // keys per histogram workgroup
     size_t tmp_elems = 0;     // keys in multi-tile segments (ping-pong buffer length)
     size_t table_bytes = 0;   // descriptor tables, uploaded at the start of scratch
     size_t scratch_bytes = 0; // tables + lookback state + histograms
