@@ -41,35 +41,55 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     T* lds = (T*)s_raw[wave];
     ST* ldss = (ST*)s_raw[wave];
 
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        int i = k * 64 + (int)lane;
-        if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)in[wave_base + i];
-    }
-    wave_sync();
-    int nvalid = ITEMS;
-    if (!full) {
-        size_t first = wave_base + (size_t)lane * ITEMS;
-        nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
-    }
+    // contiguous aligned views: 16-byte vector loads of each lane's run
+    constexpr bool DVI = DirectVec<In, T, ITEMS>::value;
+    constexpr bool DVS = DirectVec<Sten, ST, ITEMS>::value;
+    bool direct_in = false, direct_st = false;
+    if constexpr (DVI) direct_in = full && (((size_t)Contig<In>::ptr(in)) & 15) == 0;
+    if constexpr (DVS) direct_st = full && a.has_stencil && (((size_t)Contig<Sten>::ptr(sten)) & 15) == 0;
     T x[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++)
-        if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
-    unsigned flags = 0;
-    unsigned cnt = 0;
-    if (a.has_stencil) {
-        wave_sync();
+    int nvalid = ITEMS;
+    if constexpr (DVI) {
+        if (direct_in) load_run_vec<ITEMS>(Contig<In>::ptr(in) + wave_base + (size_t)lane * ITEMS, x);
+    }
+    if (!direct_in) {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             int i = k * 64 + (int)lane;
-            if (full || wave_base + i < n) ldss[WaveTile<ITEMS>::pad(i)] = sten[wave_base + i];
+            if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)in[wave_base + i];
         }
         wave_sync();
+        if (!full) {
+            size_t first = wave_base + (size_t)lane * ITEMS;
+            nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++)
+            if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
+    }
+    unsigned flags = 0;
+    unsigned cnt = 0;
+    if (a.has_stencil) {
+        ST sv[ITEMS];
+        if constexpr (DVS) {
+            if (direct_st) load_run_vec<ITEMS>(Contig<Sten>::ptr(sten) + wave_base + (size_t)lane * ITEMS, sv);
+        }
+        if (!direct_st) {
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                int i = k * 64 + (int)lane;
+                if (full || wave_base + i < n) ldss[WaveTile<ITEMS>::pad(i)] = sten[wave_base + i];
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++)
+                if (j < nvalid) sv[j] = ldss[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
+        }
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             if (j < nvalid) {
-                bool f = (bool)pred(ldss[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)]);
+                bool f = (bool)pred(sv[j]);
                 flags |= (unsigned)f << j;
                 cnt += f;
             }

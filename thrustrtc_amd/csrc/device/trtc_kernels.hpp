@@ -44,23 +44,28 @@ extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_iota_u32(unsigned*
 }
 #endif
 
+// Single scans and compactions use 512-thread tiles (8192 elements of 4
+// bytes): half as many lookbacks as 256-thread tiles.
+#define TRTC_SCAN_BLOCK 512
+
 template <class Src, class Out, class Op, class T>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_scan_k(Src src, Out out, Op op, T init, trtc::ScanArgs a)
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK) trtc_scan_k(Src src, Out out, Op op, T init, trtc::ScanArgs a)
 {
-    trtc::scan_tile<TRTC_BLOCK, trtc::ScanItems<T>::value, T>(src, out, op, init, a);
+    trtc::scan_tile<TRTC_SCAN_BLOCK, trtc::ScanItems<T>::value, T>(src, out, op, init, a);
 }
 
 template <class Src, class Op, class T>
 __global__ void __launch_bounds__(TRTC_BLOCK) trtc_reduce_k(Src src, Op op, T init, trtc::ReduceArgs a)
 {
-    trtc::reduce_body<TRTC_BLOCK, trtc::ReduceItems<T>::value, T>(src, op, init, a);
+    trtc::ReduceDispatch<TRTC_BLOCK, Src, Op, T>::template run<trtc::ReduceItems<T>::value>(src, op, init, a);
 }
 
 template <class In, class Sten, class Out, class Pred>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_copy_if_k(In in, Sten sten, Out out, Pred pred, trtc::CopyIfArgs a)
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
+trtc_copy_if_k(In in, Sten sten, Out out, Pred pred, trtc::CopyIfArgs a)
 {
     typedef typename In::value_t T;
-    trtc::copy_if_tile<TRTC_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, out, pred, a);
+    trtc::copy_if_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, out, pred, a);
 }
 
 template <class Keys, class Vals, class Out, class Pred, class Op, class V>

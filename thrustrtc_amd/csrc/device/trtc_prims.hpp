@@ -219,6 +219,57 @@ template <class T> struct PtrView
     TRTC_DEV T& operator[](size_t i) const { return p[i]; }
 };
 
+// Raw element pointer of contiguous views (VectorView, PtrView), for the
+// direct vector-load paths.
+template <class T> struct NoConst { typedef T type; };
+template <class T> struct NoConst<const T> { typedef T type; };
+template <class A, class B> struct SameType { static constexpr bool value = false; };
+template <class A> struct SameType<A, A> { static constexpr bool value = true; };
+
+template <class View> struct Contig { static constexpr bool ok = false; typedef void elem; };
+template <class U> struct Contig<VectorView<U>>
+{
+    static constexpr bool ok = true;
+    typedef U elem;
+    static TRTC_DEV U* ptr(const VectorView<U>& v) { return v._data; }
+};
+template <class U> struct Contig<PtrView<U>>
+{
+    static constexpr bool ok = true;
+    typedef U elem;
+    static TRTC_DEV U* ptr(const PtrView<U>& v) { return v.p; }
+};
+
+// True when a view holds T contiguously and T packs 16-byte vectors (4- and
+// 8-byte scalars): a lane can then load its ITEMS consecutive elements with
+// ITEMS*sizeof(T)/16 vector loads instead of a transpose through LDS.
+template <class View, class T, int ITEMS> struct DirectVec
+{
+    static constexpr bool value = Contig<View>::ok &&
+                                  SameType<typename NoConst<typename Contig<View>::elem>::type, T>::value &&
+                                  (sizeof(T) == 4 || sizeof(T) == 8) && (ITEMS * sizeof(T)) % 16 == 0;
+};
+
+typedef unsigned trtc_v4u __attribute__((ext_vector_type(4)));
+
+template <int ITEMS, class T> TRTC_DEV void load_run_vec(const T* p, T (&x)[ITEMS])
+{
+    constexpr int NV = ITEMS * (int)sizeof(T) / 16;
+    trtc_v4u v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) v[j] = __builtin_nontemporal_load((const trtc_v4u*)p + j);
+    __builtin_memcpy(&x[0], &v[0], ITEMS * sizeof(T));
+}
+
+template <int ITEMS, class T> TRTC_DEV void store_run_vec(T* p, const T (&y)[ITEMS])
+{
+    constexpr int NV = ITEMS * (int)sizeof(T) / 16;
+    trtc_v4u v[NV];
+    __builtin_memcpy(&v[0], &y[0], ITEMS * sizeof(T));
+#pragma unroll
+    for (int j = 0; j < NV; j++) __builtin_nontemporal_store(v[j], (trtc_v4u*)p + j);
+}
+
 // ---------------------------------------------------------------------------
 // Wave-local striped <-> blocked transposition through LDS.
 // Striped: lane l owns elements k*64 + l.  Blocked: lane l owns elements

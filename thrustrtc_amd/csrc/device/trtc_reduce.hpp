@@ -139,6 +139,159 @@ TRTC_DEV void reduce_body(Src& src, Op& op, const T& init, const ReduceArgs& a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming path for built-in commutative ops (Plus, Maximum, Minimum,
+// Multiplies) on contiguous arithmetic data.  The reference's own tree
+// (general_reduce.cpp:64-75) combines element i with element i+s, i.e. it
+// already relies on commutativity; here the order is fixed by n, the grid
+// and the data's alignment: each thread folds its grid-strided 16-byte
+// vectors into UNROLL accumulators, then threads, waves and workgroups are
+// combined in a fixed tree.  Loads are nontemporal and UNROLL deep per
+// thread, so the kernel streams at close to copy bandwidth.
+// ---------------------------------------------------------------------------
+template <class T> struct IsArith { static constexpr bool value = false; };
+template <> struct IsArith<int8_t> { static constexpr bool value = true; };
+template <> struct IsArith<uint8_t> { static constexpr bool value = true; };
+template <> struct IsArith<int16_t> { static constexpr bool value = true; };
+template <> struct IsArith<uint16_t> { static constexpr bool value = true; };
+template <> struct IsArith<int32_t> { static constexpr bool value = true; };
+template <> struct IsArith<uint32_t> { static constexpr bool value = true; };
+template <> struct IsArith<int64_t> { static constexpr bool value = true; };
+template <> struct IsArith<uint64_t> { static constexpr bool value = true; };
+template <> struct IsArith<float> { static constexpr bool value = true; };
+template <> struct IsArith<double> { static constexpr bool value = true; };
+
+template <class Op> struct IsCommutative { static constexpr bool value = false; };
+template <> struct IsCommutative<Plus> { static constexpr bool value = true; };
+template <> struct IsCommutative<Maximum> { static constexpr bool value = true; };
+template <> struct IsCommutative<Minimum> { static constexpr bool value = true; };
+template <> struct IsCommutative<Multiplies> { static constexpr bool value = true; };
+
+template <class Src, class Op, class T> struct StreamingReduce { static constexpr bool value = false; };
+template <class Op, class T> struct StreamingReduce<VectorView<T>, Op, T>
+{
+    static constexpr bool value = IsArith<T>::value && IsCommutative<Op>::value;
+};
+
+// Butterfly-free ordered tree over lanes: lane l takes lane l+d for d = 32..1;
+// the result is in lane 0.
+template <class T, class Op> TRTC_DEV Maybe<T> wave_tree_reduce(Maybe<T> x, Op& op)
+{
+    const unsigned l = lane_id();
+#pragma unroll
+    for (unsigned d = 32; d >= 1; d >>= 1) {
+        T yv = shfl_down(x.v, d);
+        unsigned yh = shfl_down(x.has, d);
+        if (l + d < 64 && yh) {
+            x.v = x.has ? (T)op(x.v, yv) : yv;
+            x.has = 1;
+        }
+    }
+    return x;
+}
+
+template <int BLOCK, class T, class Op>
+TRTC_DEV void reduce_streaming(const T* p, Op& op, const T& init, const ReduceArgs& a)
+{
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int V = 16 / sizeof(T);
+    constexpr int UNROLL = 4;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    __shared__ Maybe<T> s_w[WAVES];
+    __shared__ bool s_last;
+
+    const size_t n = a.n;
+    const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t nthreads = (size_t)gridDim.x * BLOCK;
+    // elements before the first 16-byte boundary, whole vectors, the tail
+    size_t head = ((16 - ((size_t)p & 15)) & 15) / sizeof(T);
+    if (((size_t)p % sizeof(T)) != 0 || head > n) head = n;  // no aligned vectors: all scalar
+    const size_t nv = (n - head) / V;
+    const size_t tail_begin = head + nv * V;
+    const v4u* q = (const v4u*)(p + head);
+
+    Maybe<T> acc[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) acc[u].has = 0;
+    for (size_t v0 = tid; v0 < nv; v0 += nthreads * UNROLL) {
+        v4u x[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            size_t v = v0 + (size_t)u * nthreads;
+            if (v < nv) x[u] = __builtin_nontemporal_load(q + v);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            size_t v = v0 + (size_t)u * nthreads;
+            if (v < nv) {
+                T e[V];
+                __builtin_memcpy(e, &x[u], 16);
+#pragma unroll
+                for (int k = 0; k < V; k++) fold(acc[u], e[k], op);
+            }
+        }
+    }
+    // head and tail elements (grid-strided scalars)
+    for (size_t i = tid; i < head; i += nthreads) fold(acc[0], p[i], op);
+    for (size_t i = tail_begin + tid; i < n; i += nthreads) fold(acc[0], p[i], op);
+    Maybe<T> t = acc[0];
+#pragma unroll
+    for (int u = 1; u < UNROLL; u++)
+        if (acc[u].has) fold(t, acc[u].v, op);
+    t = wave_tree_reduce(t, op);
+    const unsigned wave = threadIdx.x / 64;
+    if (lane_id() == 0) s_w[wave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Maybe<T> b;
+        b.has = 0;
+        for (int w = 0; w < WAVES; w++)
+            if (s_w[w].has) fold(b, s_w[w].v, op);
+        publish<Maybe<T>>(a.partials, blockIdx.x, b);
+        unsigned d = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (d == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last || wave != 0) return;
+    Maybe<T> tot;
+    tot.has = 0;
+    const unsigned G = gridDim.x;
+    for (unsigned base = 0; base < G; base += 64) {
+        unsigned b = base + lane_id();
+        Maybe<T> pp;
+        pp.has = 0;
+        if (b < G) {
+            while (!try_read<Maybe<T>>(a.partials, b, pp)) __builtin_amdgcn_s_sleep(1);
+        }
+        Maybe<T> c = wave_tree_reduce(pp, op);
+        if (lane_id() == 0 && c.has) fold(tot, c.v, op);
+    }
+    if (lane_id() == 0) {
+        T r;
+        if (a.has_init) r = tot.has ? (T)op(init, tot.v) : init;
+        else r = tot.v;
+        *(T*)a.result = r;
+    }
+}
+
+template <int BLOCK, class Src, class Op, class T> struct ReduceDispatch
+{
+    template <int ITEMS> TRTC_DEV static void run(Src& src, Op& op, const T& init, const ReduceArgs& a)
+    {
+        reduce_body<BLOCK, ITEMS, T>(src, op, init, a);
+    }
+};
+template <int BLOCK, class Op, class T> struct ReduceDispatch<BLOCK, VectorView<T>, Op, T>
+{
+    template <int ITEMS> TRTC_DEV static void run(VectorView<T>& src, Op& op, const T& init, const ReduceArgs& a)
+    {
+        if constexpr (StreamingReduce<VectorView<T>, Op, T>::value)
+            reduce_streaming<BLOCK, T>(src._data, op, init, a);
+        else
+            reduce_body<BLOCK, ITEMS, T>(src, op, init, a);
+    }
+};
+
 // Batch of independent reductions (no init): segment s is cut into tiles of
 // TILE elements, each wave of a tile reduces a contiguous quarter-range in
 // order, the tile combines its waves in order and publishes a partial, and

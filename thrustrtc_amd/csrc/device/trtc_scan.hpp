@@ -47,6 +47,20 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
     const bool full = tile_base + TILE <= n;
     T* lds = s_x[wave];
 
+    // contiguous aligned views: each lane loads its ITEMS consecutive
+    // elements with 16-byte vector loads (no LDS transpose on the way in)
+    constexpr bool DV = DirectVec<Src, T, ITEMS>::value && DirectVec<Out, T, ITEMS>::value;
+    bool direct = false;
+    if constexpr (DV) {
+        const size_t pa = (size_t)Contig<Src>::ptr(src), pb = (size_t)Contig<Out>::ptr(out);
+        direct = full && ((pa | pb) & 15) == 0;
+    }
+    T x[ITEMS];
+    int nvalid = ITEMS;
+    if constexpr (DV) {
+        if (direct) load_run_vec<ITEMS>(Contig<Src>::ptr(src) + wave_base + (size_t)lane * ITEMS, x);
+    }
+    if (!direct) {
     // striped load -> LDS
     if (full) {
 #pragma unroll
@@ -64,15 +78,14 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
     wave_sync();
 
     // blocked read: lane holds elements lane*ITEMS .. lane*ITEMS+ITEMS-1
-    int nvalid = ITEMS;
     if (!full) {
         size_t first = wave_base + (size_t)lane * ITEMS;
         nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
     }
-    T x[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; j++)
         if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
+    }
 
     // thread aggregate
     T tagg = x[0];
@@ -169,7 +182,7 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
         }
     }
 
-    // blocked -> LDS -> striped stores
+    // blocked -> LDS -> striped stores (full cache lines per store instruction)
     wave_sync();
 #pragma unroll
     for (int j = 0; j < ITEMS; j++)

@@ -18,6 +18,7 @@ using namespace trtc_host;
 namespace {
 
 constexpr unsigned kBlock = 256;
+constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: trtc_scan_k and trtc_copy_if_k
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -250,7 +251,7 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kBlock * items_for(tsz);
+    size_t tile = (size_t)kScanBlock * items_for(tsz);
     size_t tiles = (n + tile - 1) / tile;
     LookbackScratch lb;
     if (!lb.make(tiles, tsz, s)) return false;
@@ -261,7 +262,7 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     args.view(op);
     args.raw(initv.data(), initv.size());
     args.val(a);
-    return launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s);
 }
 
 // ---------------------------------------------------------------- reduce
@@ -287,7 +288,9 @@ bool general_reduce(const DVVectorLike& vec, const DeviceViewable* init, const F
     const size_t waves_per_block = kBlock / 64;
     const size_t chunk = 64 * (size_t)items_for(tsz);
     size_t chunks = (n + chunk - 1) / chunk;
-    size_t max_blocks = (size_t)std::max(1, context().num_cus) * 8;
+    // four 256-thread workgroups per CU: every workgroup of either kernel
+    // path (ordered or streaming) is resident at once
+    size_t max_blocks = (size_t)std::max(1, context().num_cus) * 4;
     size_t blocks = std::min(max_blocks, (chunks + waves_per_block - 1) / waves_per_block);
     size_t waves = blocks * waves_per_block;
     size_t per_wave = (chunks + waves - 1) / waves * chunk;
@@ -329,7 +332,7 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return (uint32_t)-1;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kBlock * items_for(vin.elem_size());
+    size_t tile = (size_t)kScanBlock * items_for(vin.elem_size());
     size_t tiles = (n + tile - 1) / tile;
     LookbackScratch lb;
     if (!lb.make(tiles, 8, s)) return (uint32_t)-1;
@@ -343,7 +346,7 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
     args.view(vout);
     args.view(pred);
     args.val(a);
-    if (!launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s)) return (uint32_t)-1;
+    if (!launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s)) return (uint32_t)-1;
     unsigned long long count = 0;
     if (!check_hip(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return (uint32_t)-1;
     if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return (uint32_t)-1;
