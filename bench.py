@@ -168,7 +168,7 @@ def leg_single_pass(trtc, native, wl, a, log):
     return res
 
 
-def leg_small_calls(trtc, native, wl, log):
+def leg_small_calls(trtc, native, wl, log, cpu=True):
     """Latency of single small calls: TRTC_Sort on 2^20 uint32 (seed 0) and
     TRTC_Inclusive_Scan on 2^20 int32 in [-100, 100] (seed 0); median over
     100 calls, input restored by a device copy outside the timed call."""
@@ -209,6 +209,24 @@ def leg_small_calls(trtc, native, wl, log):
         ok = "%016x" % wl._lib.wl_fnv1a64(out.ctypes.data, out.nbytes) == gold["inclusive_scan_i32_2p20_seed0"]["out"]
     res["inclusive_scan_i32"] = {"median_us": round(1e6 * float(np.median(lat)), 1),
                                  "p99_us": round(1e6 * float(np.percentile(lat, 99)), 1), "bit_exact": ok}
+    if cpu:
+        from helpers import Oracle
+        L = Oracle().L
+        ts, tc = [], []
+        for i in range(51):
+            kk = k.copy()
+            t0 = time.perf_counter()
+            L.bl_sort_u32(kk.ctypes.data, n)
+            ts.append(time.perf_counter() - t0)
+            yy = np.empty_like(x)
+            t0 = time.perf_counter()
+            L.bl_inclusive_scan_i32(x.ctypes.data, yy.ctypes.data, n)
+            tc.append(time.perf_counter() - t0)
+        res["cpu"] = {"cores": int(L.bl_threads()), "kind": "port",
+                      "sort_u32_median_us": round(1e6 * float(np.median(ts[1:])), 1),
+                      "inclusive_scan_i32_median_us": round(1e6 * float(np.median(tc[1:])), 1),
+                      "sample": "the same 2^20 inputs, median of 50 calls after one warm-up "
+                                "(__gnu_parallel::stable_sort, __gnu_parallel::partial_sum)"}
     log(f"small calls: {res}")
     return res
 
@@ -297,6 +315,17 @@ def leg_segmented(trtc, native, wl, a, log):
         perm = orc.sort_perm(keys[lo:hi], "uint32_t")
         spot = spot and bool((gk[lo:hi] == keys[lo:hi][perm]).all()) and bool((gv[lo:hi] == perm + lo).all())
         spot = spot and bool((gy[lo:hi] == orc.inclusive_scan(x[lo:hi], "int32_t")).all())
+    cpu = None
+    if not a.no_cpu:
+        kk, vv, yy = keys.copy(), np.arange(total, dtype=np.uint32), np.empty_like(x)
+        t0 = time.perf_counter()
+        orc.L.bl_segmented_sort_pairs_u32(kk.ctypes.data, vv.ctypes.data, offs.ctypes.data, len(sizes))
+        orc.L.bl_segmented_inclusive_scan_i32(x.ctypes.data, yy.ctypes.data, offs.ctypes.data, len(sizes))
+        dt = time.perf_counter() - t0
+        cpu = {"ms": round(dt * 1e3, 1), "cores": int(orc.L.bl_threads()), "kind": "port",
+               "sample": "the full workload once: OpenMP over segments, std::stable_sort of (key, value) pairs "
+                         "and std::inclusive_scan per segment"}
+        log(f"segmented cpu baseline: {cpu}")
     res = {"segments": int(len(sizes)), "total_elems": int(total),
            "sizes": "log-uniform [16, 2^20], mt19937_64 seed 7",
            "batched_graph_ms": round(batched_ms, 3), "batched_wall_ms": round(1e3 * float(np.median(walls)), 3),
@@ -305,7 +334,7 @@ def leg_segmented(trtc, native, wl, a, log):
            "pct_hbm_68N_plus_8N": round(100 * (76.0 * total) / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
            "instantiate_s_incl_tuning": round(inst_s, 2),
            "batch": b.describe(), "tuning": trtc.Batch_Tuning_Report(),
-           "batched_equals_loop_bitwise": same, "oracle_spot_check_32_segments": spot}
+           "batched_equals_loop_bitwise": same, "oracle_spot_check_32_segments": spot, "cpu": cpu}
     log(f"segmented: {res}")
     return res
 
@@ -420,7 +449,7 @@ def main():
         if a.only == "single_pass":
             res = leg_single_pass(trtc, native, wl, a, log)
         elif a.only == "small":
-            res = leg_small_calls(trtc, native, wl, log)
+            res = leg_small_calls(trtc, native, wl, log, not a.no_cpu)
         elif a.only == "served":
             res = leg_served(trtc, native, wl, a, log)
         else:
@@ -500,6 +529,19 @@ def main():
         if t.get("n") == N and pass_name.startswith(t.get("kernel", "?")[:-1]):
             traffic = round(t["read_bytes"] + t["write_bytes"])
 
+    # the per-GPU shape of an 8-way sharded 2^28 sort: 2^25 pairs
+    shard = None
+    if a.n == 28 and not a.no_variants:
+        ks = wl.u32_uniform(1 << 25, 2)
+        N_save = N
+        N = 1 << 25
+        s3, _, ok3 = run(ks, K, W, check=True)
+        N = N_save
+        shard = {"n": 1 << 25, "gkeys_per_s": round((1 << 25) / s3 / 1e9, 3), "ms_per_step": round(s3 * 1e3, 4),
+                 "pct_hbm_68N": round(100 * 68.0 * (1 << 25) / s3 / 1e9 / HBM_PEAK_GBS, 2), "bit_exact_check": ok3}
+        log(f"shard 2^25: {shard}")
+        del ks
+
     variants = {}
     if not a.no_variants:
         for name, gen in [("u32_low_entropy", lambda: wl.u32_low_entropy(N, 2)),
@@ -530,7 +572,7 @@ def main():
     segmented = None if a.no_segmented else leg_segmented(trtc, native, wl, a, log)
     served = None if a.no_served else leg_served(trtc, native, wl, a, log)
     single_pass = None if a.no_single_pass else leg_single_pass(trtc, native, wl, a, log)
-    small = None if a.no_small else leg_small_calls(trtc, native, wl, log)
+    small = None if a.no_small else leg_small_calls(trtc, native, wl, log, not a.no_cpu)
     copy_bw = leg_copy(trtc, N, 5, 2)
     log(f"copy bandwidth: {copy_bw}")
 
@@ -557,6 +599,7 @@ def main():
         "model_68N": {"gbs": round(model_gbs, 1), "pct_hbm": round(100 * model_gbs / HBM_PEAK_GBS, 2)},
         "kernels": prof,
         "variants": variants,
+        "shard_2p25": shard,
         "cpu_baseline": cpu,
         "single_pass_2p29": single_pass,
         "batched_segments": segmented,

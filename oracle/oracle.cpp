@@ -454,4 +454,31 @@ uint64_t bl_copy_if_mod3_i32(const int32_t* in, int32_t* out, uint64_t n)
     return cnt[nt];
 }
 
+// Batched segments on the host: OpenMP over segments (dynamic schedule, the
+// sizes are uneven), std::stable_sort of (key, value) pairs and
+// std::inclusive_scan (uint32 wrap) per segment.
+void bl_segmented_sort_pairs_u32(uint32_t* keys, uint32_t* values, const uint64_t* offsets, uint64_t num_segments)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < (int64_t)num_segments; s++) {
+        uint64_t b = offsets[s], e = offsets[s + 1];
+        std::vector<KV> kv(e - b);
+        for (uint64_t i = b; i < e; i++) kv[i - b] = KV{keys[i], values[i]};
+        std::stable_sort(kv.begin(), kv.end(), [](const KV& x, const KV& y) { return x.k < y.k; });
+        for (uint64_t i = b; i < e; i++) {
+            keys[i] = kv[i - b].k;
+            values[i] = kv[i - b].v;
+        }
+    }
+}
+
+void bl_segmented_inclusive_scan_i32(const int32_t* in, int32_t* out, const uint64_t* offsets, uint64_t num_segments)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < (int64_t)num_segments; s++) {
+        uint64_t b = offsets[s], e = offsets[s + 1];
+        std::inclusive_scan((const uint32_t*)in + b, (const uint32_t*)in + e, (uint32_t*)out + b);
+    }
+}
+
 }  // extern "C"

@@ -79,6 +79,8 @@ class Oracle:
         L.bl_reduce_f32.restype = ctypes.c_float
         L.bl_copy_if_mod3_i32.argtypes = [vp, vp, u64]
         L.bl_copy_if_mod3_i32.restype = u64
+        L.bl_segmented_sort_pairs_u32.argtypes = [vp, vp, vp, u64]
+        L.bl_segmented_inclusive_scan_i32.argtypes = [vp, vp, vp, u64]
         self.L = L
 
     def sort(self, keys, elem, descending=False):

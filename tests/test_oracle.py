@@ -248,3 +248,22 @@ def test_generators_pinned(wl):
     assert x.min() == -1000 and x.max() == 1000
     sizes, total = wl.segment_sizes(4096, 7, 16, 1 << 20)
     assert sizes.min() >= 16 and sizes.max() <= 1 << 20 and total == int(sizes.sum())
+
+
+def test_host_baseline_segmented_matches_oracle(orc, wl):
+    """The segmented host baseline computes per-segment stable sorts and scans."""
+    sizes = np.array([0, 1, 5, 1000, 37, 4096, 2], dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(offs[-1])
+    keys = wl.i32_range(n, 3, 0, 20).astype(np.uint32)  # many ties: stability matters
+    vals = np.arange(n, dtype=np.uint32)
+    k2, v2 = keys.copy(), vals.copy()
+    orc.L.bl_segmented_sort_pairs_u32(k2.ctypes.data, v2.ctypes.data, offs.ctypes.data, len(sizes))
+    x = wl.i32_range(n, 4, -1000, 1000)
+    y = np.empty_like(x)
+    orc.L.bl_segmented_inclusive_scan_i32(x.ctypes.data, y.ctypes.data, offs.ctypes.data, len(sizes))
+    for s in range(len(sizes)):
+        b, e = int(offs[s]), int(offs[s + 1])
+        perm = orc.sort_perm(keys[b:e], "uint32_t")
+        assert (k2[b:e] == keys[b:e][perm]).all() and (v2[b:e] == perm + b).all()
+        assert (y[b:e] == orc.inclusive_scan(x[b:e], "int32_t")).all()
