@@ -128,6 +128,39 @@ def test_sort_by_key_parity(trtc, orc, wl, kind, n, greater):
     assert (gv == perm.astype(np.uint32)).all()
 
 
+# Multi-million-key inputs: thousands of tiles per pass, long lookback walks,
+# a partial last tile, skewed digits.
+@pytest.mark.parametrize("kind", ["u32", "u32_low", "f32", "i32"])
+@pytest.mark.parametrize("n", [1 << 22, (1 << 22) + 12345, 3 << 21])
+@pytest.mark.parametrize("greater", [False, True])
+def test_sort_by_key_large(trtc, orc, wl, kind, n, greater):
+    say("sort_by_key large", kind, n, greater)
+    keys, elem = _keys(wl, kind, n, 5)
+    idx = np.arange(n, dtype=np.uint32)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(idx)
+    trtc.Sort_By_Key(dk, dv, trtc.Greater() if greater else trtc.Less())
+    perm = orc.sort_perm(keys, elem, greater)
+    gk, gv = dk.to_host(), dv.to_host()
+    assert (gk.view(np.uint32) == keys[perm].view(np.uint32)).all()
+    assert (gv == perm.astype(np.uint32)).all()
+
+
+def test_sort_large_keys_only_and_16bit(trtc, orc, wl):
+    say("sort large keys-only u32 and 16-bit keys")
+    n = (1 << 22) + 777
+    keys, elem = _keys(wl, "u32_low", n, 9)
+    dk = trtc.device_vector_from_numpy(keys)
+    trtc.Sort(dk)
+    assert (dk.to_host() == orc.sort(keys, elem)).all()
+    k16 = (wl.u32_uniform(n, 10) & 0xFFFF).astype(np.uint16)
+    d16 = trtc.device_vector_from_numpy(k16)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(d16, dv)
+    perm = orc.sort_perm(k16, "uint16_t")
+    assert (d16.to_host() == k16[perm]).all() and (dv.to_host() == perm.astype(np.uint32)).all()
+
+
 @pytest.mark.parametrize("elem", ["int8_t", "uint8_t", "int16_t", "uint16_t", "int64_t", "uint64_t", "double",
                                   "float", "int32_t", "uint32_t"])
 def test_sort_types(trtc, orc, wl, elem):
