@@ -128,3 +128,13 @@ def test_embedded_headers_match_sources():
     for h in ["trtc_builtin.hpp", "trtc_prims.hpp", "trtc_radix.hpp", "trtc_kernels.hpp"]:
         src = open(os.path.join(ROOT, "thrustrtc_amd", "csrc", "device", h)).read()
         assert src in inc, h
+
+
+def test_merge_sort_kernels_compile_for_user_comparator(nm):
+    # the comparison-sort kernels instantiate on a user comparator functor
+    n = nm.native
+    f = n.n_functor_create(_ptrs(nm, []), _strs(nm, []), _strs(nm, ["x", "y"]), b"        return (x & 255) < (y & 255);\n")
+    name = n.n_dv_name_view_cls(f).decode()
+    for k in ("trtc_msort_tile_k", "trtc_msort_merge_k"):
+        assert n.n_jit_compile_check(b"gfx950", f"{k}<uint32_t, {name}>".encode()) == 0
+    assert n.n_jit_compile_check(b"gfx950", b"trtc_msort_merge_k<double, Greater>") == 0

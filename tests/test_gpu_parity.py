@@ -420,3 +420,50 @@ def test_bad_functor_is_an_error_not_a_crash(trtc):
         trtc.Copy_If(d, o, bad)
     good = trtc.Functor({}, ["x"], "        return x > 1;\n")
     assert trtc.Copy_If(d, o, good) == 2
+
+
+# ---------------------------------------------------------------- comparison sort
+MSORT_SIZES = [1, 2, 100, 2047, 2048, 2049, 10000, 100003, 1 << 20]
+
+
+@pytest.mark.parametrize("n", MSORT_SIZES)
+def test_sort_user_comparator(trtc, wl, n):
+    """A user comparator takes the stable merge path: low byte of the key only,
+    so most keys tie and stability decides the order."""
+    say("merge sort low-byte comparator", n)
+    keys = wl.u32_uniform(n, 13)
+    comp = trtc.Functor({}, ["x", "y"], "        return (x & 255) < (y & 255);\n")
+    d = trtc.device_vector_from_numpy(keys)
+    trtc.Sort(d, comp)
+    exp = keys[np.argsort(keys & 255, kind="stable")]
+    assert (d.to_host() == exp).all()
+
+
+@pytest.mark.parametrize("n", [5, 2049, 100003, 1 << 20])
+def test_sort_by_key_user_comparator_values(trtc, orc, wl, n):
+    """Descending by a user functor (not the built-in Greater) with int32 keys,
+    and float keys by magnitude with int16 values (payload gathered)."""
+    say("merge sort_by_key user comparators", n)
+    keys = wl.i32_range(n, 14, -500, 500)
+    idx = np.arange(n, dtype=np.uint32)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(idx)
+    trtc.Sort_By_Key(dk, dv, trtc.Functor({}, ["x", "y"], "        return x > y;\n"))
+    perm = orc.sort_perm(keys, "int32_t", True)
+    assert (dk.to_host() == keys[perm]).all() and (dv.to_host() == perm.astype(np.uint32)).all()
+    fk = wl.f32_normal_signed_zeros(n, 15, 16)
+    vals = (np.arange(n) % 30000).astype(np.int16)
+    dfk = trtc.device_vector_from_numpy(fk)
+    dvv = trtc.device_vector_from_numpy(vals)
+    trtc.Sort_By_Key(dfk, dvv, trtc.Functor({}, ["x", "y"],
+                                            "        return (x < 0 ? -x : x) < (y < 0 ? -y : y);\n"))
+    p2 = np.argsort(np.abs(fk), kind="stable")
+    assert (dfk.to_host().view(np.uint32) == fk[p2].view(np.uint32)).all()
+    assert (dvv.to_host() == vals[p2]).all()
+
+
+def test_sort_user_comparator_known_answer(trtc):
+    # docs/QuickStartGuide.md (Sort with a comparator): {1,4,2,8,5,7} descending
+    d = trtc.device_vector_from_list([1, 4, 2, 8, 5, 7], "int32_t")
+    trtc.Sort(d, trtc.Functor({}, ["x", "y"], "        return x > y;\n"))
+    assert d.to_host().tolist() == [8, 7, 5, 4, 2, 1]

@@ -36,6 +36,7 @@ DEVICE_HEADERS = [
     "trtc_copy_if.hpp",
     "trtc_scan_by_key.hpp",
     "trtc_radix.hpp",
+    "trtc_msort.hpp",
     "trtc_kernels.hpp",
 ]
 

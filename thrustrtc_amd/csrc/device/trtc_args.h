@@ -76,6 +76,13 @@ struct RadixPassArgs
     int descending;
 };
 
+struct MergeSortArgs
+{
+    size_t n;
+    size_t width;  // run length of the input of a merge pass
+    int with_idx;  // carry the input index of every key
+};
+
 // ---- batched segments: many independent sequences in one launch --------
 // A launch covers the tiles of a list of segments; tile ids 0..num_tiles-1
 // are handed out by an atomic counter in order, and tile_seg maps each one to

@@ -12,6 +12,7 @@
 #include "trtc_copy_if.hpp"
 #include "trtc_scan_by_key.hpp"
 #include "trtc_radix.hpp"
+#include "trtc_msort.hpp"
 
 #define TRTC_BLOCK 256
 
@@ -123,4 +124,21 @@ template <class Op, class T>
 __global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_reduce_k(Op op, trtc::SegReduceArgs a)
 {
     trtc::seg_reduce_tile<TRTC_BLOCK, trtc::ReduceItems<T>::value, TRTC_SEG_REDUCE_TILE, T>(op, a);
+}
+
+// ---- stable comparison sort (user comparators) ----
+#define TRTC_MSORT_BLOCK 256
+#define TRTC_MSORT_ITEMS 8
+template <class T, class Comp>
+__global__ void __launch_bounds__(TRTC_MSORT_BLOCK) trtc_msort_tile_k(T* keys, unsigned* idx, Comp comp,
+                                                                     trtc::MergeSortArgs a)
+{
+    trtc::msort_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, T>(keys, idx, comp, a);
+}
+
+template <class T, class Comp>
+__global__ void __launch_bounds__(TRTC_MSORT_BLOCK)
+trtc_msort_merge_k(const T* kin, const unsigned* iin, T* kout, unsigned* iout, Comp comp, trtc::MergeSortArgs a)
+{
+    trtc::msort_merge<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, T>(kin, iin, kout, iout, comp, a);
 }

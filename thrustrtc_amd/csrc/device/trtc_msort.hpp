@@ -1,0 +1,218 @@
+// Stable comparison sort for user comparators (the reference's merge sort,
+// merge_sort.cpp: ties keep input order, i.e. std::stable_sort with comp).
+//
+// 1. Tile sort: a tile of BLOCK*ITEMS keys (with their input indices) is
+//    sorted in LDS: every thread sorts its ITEMS consecutive keys (odd-even
+//    transposition, stable), then runs are merged pairwise up to the tile size; a
+//    key's new position is its offset in its own run plus its rank in the
+//    partner run (lower bound for the left run, upper bound for the right
+//    run, so equal keys keep their order).
+// 2. Merge passes: runs of width w are merged pairwise into runs of 2w.  Each
+//    workgroup produces one output tile: it finds where the tile's first and
+//    last output diagonals cut the two input runs (merge-path binary search,
+//    ties to the left run), stages both input ranges in LDS, and every
+//    thread merges its ITEMS outputs from its own split.
+#pragma once
+#include "trtc_prims.hpp"
+
+namespace trtc {
+
+// Number of a[0..len) that go before x (strictly less: comp(a, x)).
+template <class T, class Comp>
+TRTC_DEV unsigned lower_rank(const T* a, unsigned len, const T& x, Comp& comp)
+{
+    unsigned lo = 0, hi = len;
+    while (lo < hi) {
+        unsigned mid = (lo + hi) >> 1;
+        if (comp(a[mid], x)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Number of a[0..len) that go before or with x (not greater: !comp(x, a)).
+template <class T, class Comp>
+TRTC_DEV unsigned upper_rank(const T* a, unsigned len, const T& x, Comp& comp)
+{
+    unsigned lo = 0, hi = len;
+    while (lo < hi) {
+        unsigned mid = (lo + hi) >> 1;
+        if (!comp(x, a[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int BLOCK, int ITEMS, class T, class Comp>
+TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs& a)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ T s_k[TILE];
+    __shared__ unsigned s_i[TILE];
+    const size_t base = (size_t)blockIdx.x * TILE;
+    const unsigned valid = (unsigned)((a.n - base) < (size_t)TILE ? (a.n - base) : (size_t)TILE);
+
+    for (unsigned p = threadIdx.x; p < valid; p += BLOCK) {
+        s_k[p] = keys[base + p];
+        s_i[p] = a.with_idx ? idx[base + p] : 0u;
+    }
+    __syncthreads();
+    // 1. each thread: stable sort of its ITEMS consecutive keys (odd-even
+    //    transposition: adjacent swaps of strict inversions only, constant
+    //    register indices)
+    T k[ITEMS];
+    unsigned ix[ITEMS];
+    const unsigned p0 = threadIdx.x * ITEMS;
+    const unsigned mine = p0 >= valid ? 0 : (valid - p0 < (unsigned)ITEMS ? valid - p0 : (unsigned)ITEMS);
+#pragma unroll
+    for (int e = 0; e < ITEMS; e++)
+        if ((unsigned)e < mine) {
+            k[e] = s_k[p0 + e];
+            ix[e] = s_i[p0 + e];
+        }
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+#pragma unroll
+        for (int e = r & 1; e + 1 < ITEMS; e += 2) {
+            if ((unsigned)(e + 1) < mine && comp(k[e + 1], k[e])) {
+                T t = k[e];
+                k[e] = k[e + 1];
+                k[e + 1] = t;
+                unsigned ti = ix[e];
+                ix[e] = ix[e + 1];
+                ix[e + 1] = ti;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < ITEMS; e++)
+        if ((unsigned)e < mine) {
+            s_k[p0 + e] = k[e];
+            s_i[p0 + e] = ix[e];
+        }
+    __syncthreads();
+    // 2. pairwise run merges inside the tile
+    for (unsigned run = ITEMS; run < (unsigned)TILE; run <<= 1) {
+        unsigned pos[ITEMS];
+#pragma unroll
+        for (int e = 0; e < ITEMS; e++) {
+            const unsigned p = p0 + e;
+            pos[e] = p;
+            if ((unsigned)e >= mine) continue;
+            k[e] = s_k[p];
+            ix[e] = s_i[p];
+            const unsigned g = p / (2 * run) * (2 * run);  // start of the run pair
+            const unsigned o = p - g;
+            if (g + run >= valid) continue;  // no right run: position unchanged
+            const unsigned rlen = (g + 2 * run <= valid ? 2 * run : valid - g) - run;
+            if (o < run)
+                pos[e] = g + o + lower_rank(s_k + g + run, rlen, k[e], comp);
+            else
+                pos[e] = g + (o - run) + upper_rank(s_k + g, run, k[e], comp);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < ITEMS; e++)
+            if ((unsigned)e < mine) {
+                s_k[pos[e]] = k[e];
+                s_i[pos[e]] = ix[e];
+            }
+        __syncthreads();
+    }
+    for (unsigned p = threadIdx.x; p < valid; p += BLOCK) {
+        keys[base + p] = s_k[p];
+        if (a.with_idx) idx[base + p] = s_i[p];
+    }
+}
+
+// Merge-path split of diagonal d of runs A (lenA) and B (lenB): the number of
+// the first d outputs taken from A (equal keys: A first).
+template <class T, class Comp>
+TRTC_DEV size_t merge_split(const T* A, size_t lenA, const T* B, size_t lenB, size_t d, Comp& comp)
+{
+    size_t lo = d > lenB ? d - lenB : 0;
+    size_t hi = d < lenA ? d : lenA;
+    while (lo < hi) {
+        size_t mid = (lo + hi) >> 1;
+        if (!comp(B[d - mid - 1], A[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int BLOCK, int ITEMS, class T, class Comp>
+TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* iout, Comp& comp,
+                          const MergeSortArgs& a)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ T s_k[TILE];
+    __shared__ unsigned s_i[TILE];
+    __shared__ size_t s_split[2];
+    const size_t n = a.n, w = a.width;
+    const size_t out0 = (size_t)blockIdx.x * TILE;
+    const size_t pair = out0 / (2 * w) * (2 * w);
+    const size_t lenA = (pair + w < n ? w : n - pair);
+    const size_t lenB = pair + w < n ? ((pair + 2 * w <= n ? 2 * w : n - pair) - w) : 0;
+    const T* A = kin + pair;
+    const T* B = kin + pair + lenA;
+    const size_t d0 = out0 - pair;
+    const size_t d1 = (d0 + TILE < lenA + lenB) ? d0 + TILE : lenA + lenB;
+    if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split(A, lenA, B, lenB, threadIdx.x ? d1 : d0, comp);
+    __syncthreads();
+    const size_t a0 = s_split[0], a1 = s_split[1];
+    const size_t b0 = d0 - a0, b1 = d1 - a1;
+    const unsigned na = (unsigned)(a1 - a0), nb = (unsigned)(b1 - b0);
+    for (unsigned p = threadIdx.x; p < na + nb; p += BLOCK) {
+        if (p < na) {
+            s_k[p] = A[a0 + p];
+            s_i[p] = a.with_idx ? iin[pair + a0 + p] : 0u;
+        } else {
+            s_k[p] = B[b0 + p - na];
+            s_i[p] = a.with_idx ? iin[pair + lenA + b0 + p - na] : 0u;
+        }
+    }
+    __syncthreads();
+    // this thread's outputs: diagonals [t*ITEMS, t*ITEMS + ITEMS) of the staged ranges
+    const unsigned total = na + nb;
+    const unsigned q0 = threadIdx.x * ITEMS;
+    T k[ITEMS];
+    unsigned ix[ITEMS];
+    unsigned cnt = 0;
+    if (q0 < total) {
+        unsigned i = (unsigned)merge_split(s_k, na, s_k + na, nb, q0, comp);
+        unsigned j = q0 - i;
+        cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
+#pragma unroll
+        for (int e = 0; e < ITEMS; e++) {
+            if ((unsigned)e >= cnt) break;
+            bool takeA;
+            if (i >= na) takeA = false;
+            else if (j >= nb) takeA = true;
+            else takeA = !comp(s_k[na + j], s_k[i]);
+            if (takeA) {
+                k[e] = s_k[i];
+                ix[e] = s_i[i];
+                i++;
+            } else {
+                k[e] = s_k[na + j];
+                ix[e] = s_i[na + j];
+                j++;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < ITEMS; e++)
+        if ((unsigned)e < cnt) {
+            s_k[q0 + e] = k[e];
+            s_i[q0 + e] = ix[e];
+        }
+    __syncthreads();
+    for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
+        kout[out0 + p] = s_k[p];
+        if (a.with_idx) iout[out0 + p] = s_i[p];
+    }
+}
+
+}  // namespace trtc

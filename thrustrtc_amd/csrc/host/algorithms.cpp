@@ -190,6 +190,64 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     return true;
 }
 
+// ---------------------------------------------------------------- merge sort
+// Stable comparison sort with any comparator functor (merge_sort.cpp:7-280):
+// tiles of 2048 keys sorted in LDS, then log2(n / 2048) merge passes; keys
+// carry their input index when values have to follow.
+constexpr size_t kMsortTile = 256 * 8;  // TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS
+
+bool merge_sort(void* keys, const std::string& T, size_t esize, size_t n, const Functor& comp, void* vals,
+                size_t vsize, hipStream_t s)
+{
+    if (n <= 1) return true;
+    const std::string cc = vcls(comp);
+    const std::string tname = "trtc_msort_tile_k<" + T + ", " + cc + ">";
+    const std::string mname = "trtc_msort_merge_k<" + T + ", " + cc + ">";
+    KernelRef tk, mk;
+    if (!get_template_kernel(tname, tname, tk)) return false;
+    const bool with_idx = vals != nullptr;
+    PoolBuf ia(with_idx ? n * 4 : 0, s), ib(with_idx ? n * 4 : 0, s), kb(n * esize, s);
+    if ((with_idx && (!ia.p || !ib.p)) || !kb.p) return false;
+    if (with_idx && !iota_u32(ia.as<unsigned>(), n, s)) return false;
+    trtc::MergeSortArgs a{n, kMsortTile, with_idx ? 1 : 0};
+    const unsigned tiles = (unsigned)((n + kMsortTile - 1) / kMsortTile);
+    {
+        Args args;
+        args.val(keys);
+        args.val(ia.p);
+        args.view(comp);
+        args.val(a);
+        if (!launch_kernel(tk, dim3(tiles), dim3(256), args.data(), 0, s)) return false;
+    }
+    void* kbuf[2] = {keys, kb.p};
+    void* ibuf[2] = {ia.p, ib.p};
+    int cur = 0;
+    if (n > kMsortTile) {
+        if (!get_template_kernel(mname, mname, mk)) return false;
+        for (size_t w = kMsortTile; w < n; w *= 2) {
+            a.width = w;
+            Args args;
+            args.val(kbuf[cur]);
+            args.val(ibuf[cur]);
+            args.val(kbuf[cur ^ 1]);
+            args.val(ibuf[cur ^ 1]);
+            args.view(comp);
+            args.val(a);
+            if (!launch_kernel(mk, dim3(tiles), dim3(256), args.data(), 0, s)) return false;
+            cur ^= 1;
+        }
+    }
+    if (cur == 1 &&
+        !check_hip(hipMemcpyAsync(keys, kb.p, n * esize, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+        return false;
+    if (!with_idx) return true;
+    PoolBuf vcopy(n * vsize, s);
+    if (!vcopy.p) return false;
+    if (!check_hip(hipMemcpyAsync(vcopy.p, vals, n * vsize, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+        return false;
+    return gather_bytes(vcopy.p, vals, (const unsigned*)ibuf[cur], n, vsize, s);
+}
+
 bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp)
 {
     if (!ensure_init()) return false;
@@ -225,10 +283,32 @@ bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp
             return false;
         return gather_bytes(vcopy.p, vp, idx.as<unsigned>(), n, vb, s);
     }
-    TRTC_ERR("Sort: comparator '%s' on element type '%s' is not supported (only Less/Greater on integer and "
-             "floating-point keys)",
-             cn.c_str(), keys.name_elem_cls().c_str());
-    return false;
+    // any other comparator or key type: stable merge sort
+    void* kp = contiguous_data(keys);
+    if (!kp) {
+        TRTC_ERR("Sort: keys must be a DVVector or DVVectorAdaptor");
+        return false;
+    }
+    size_t n = keys.size();
+    void* vp = nullptr;
+    size_t vb = 0;
+    if (values) {
+        vp = contiguous_data(*values);
+        if (!vp) {
+            TRTC_ERR("Sort_By_Key: values must be a DVVector or DVVectorAdaptor");
+            return false;
+        }
+        if (values->size() < n) {
+            TRTC_ERR("Sort_By_Key: values (%zu) shorter than keys (%zu)", values->size(), n);
+            return false;
+        }
+        vb = values->elem_size();
+    }
+    if (n >= (1ull << 32)) {
+        TRTC_ERR("Sort: comparison sort of %zu keys (limit 2^32 - 1)", n);
+        return false;
+    }
+    return merge_sort(kp, canon_type(keys.name_elem_cls()), keys.elem_size(), n, comp, vp, vb, cur_stream());
 }
 
 // ---------------------------------------------------------------- scan
