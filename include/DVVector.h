@@ -68,11 +68,14 @@ public:
     virtual bool is_readable() const { return m_readable; }
     virtual bool is_writable() const { return m_writable; }
     virtual ViewBuf view() const;
+    // device address of element 0 when the underlying vector is contiguous
+    void* contiguous() const { return m_contig; }
 
 private:
     bool m_readable, m_writable;
     ViewBuf m_view_vec_value;
     size_t m_begin, m_end;
+    void* m_contig;
 };
 
 #endif

@@ -1,0 +1,16 @@
+// Stable partition by a predicate; returns the number of items for which it holds (reference: partition.h:10-13; Partition_Point and Is_Partitioned, partition.h:14-16, are not built).
+#ifndef _TRTC_partition_h
+#define _TRTC_partition_h
+
+#include "TRTC_api.h"
+#include "TRTCContext.h"
+#include "DeviceViewable.h"
+#include "DVVector.h"
+#include "functor.h"
+
+uint32_t THRUST_RTC_API TRTC_Partition(DVVectorLike& vec, const Functor& pred);
+uint32_t THRUST_RTC_API TRTC_Partition_Stencil(DVVectorLike& vec, const DVVectorLike& stencil, const Functor& pred);
+uint32_t THRUST_RTC_API TRTC_Partition_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_true, DVVectorLike& vec_false, const Functor& pred);
+uint32_t THRUST_RTC_API TRTC_Partition_Copy_Stencil(const DVVectorLike& vec_in, const DVVectorLike& stencil, DVVectorLike& vec_true, DVVectorLike& vec_false, const Functor& pred);
+
+#endif

@@ -95,6 +95,17 @@ TRTC_CAPI void* n_dvvector_from_dvs(void* ptr_dvs);              /* api.h:82 */
 TRTC_CAPI void* n_dvvectoradaptor_create(const char* elem_cls, unsigned long long size, void* data); /* api.h:83 */
 TRTC_CAPI void* n_dvrange_create(void* ptr_in, unsigned long long begin, unsigned long long end); /* api.h:84 */
 
+/* ---- fake vectors (api.h:90-97; fake_vectors/*.h) ---- */
+TRTC_CAPI void* n_dvconstant_create(void* cptr, unsigned long long size);          /* api.h:90 */
+TRTC_CAPI void* n_dvcounter_create(void* cptr, unsigned long long size);           /* api.h:91 */
+TRTC_CAPI void* n_dvdiscard_create(const char* elem_cls, unsigned long long size); /* api.h:92 */
+TRTC_CAPI void* n_dvpermutation_create(void* ptr_value, void* ptr_index);          /* api.h:93 */
+TRTC_CAPI void* n_dvreverse_create(void* ptr_value);                               /* api.h:94 */
+TRTC_CAPI void* n_dvtransform_create(void* ptr_in, const char* elem_cls, void* ptr_op); /* api.h:95 */
+TRTC_CAPI void* n_dvzipped_create(void* ptr_vecs, void* ptr_elem_names);          /* api.h:96 */
+TRTC_CAPI void* n_dvcustomvector_create(void* ptr_dvs, void* ptr_names, const char* name_idx, const char* body,
+                                        const char* elem_cls, unsigned long long size, unsigned read_only); /* api.h:97 */
+
 /* ---- functors (api.h:100-101) ---- */
 TRTC_CAPI void* n_functor_create(void* ptr_dvs, void* ptr_names, void* ptr_functor_params, const char* code_body); /* api.h:100 */
 TRTC_CAPI void* n_built_in_functor_create(const char* name_built_in_view_cls); /* api.h:101 */
@@ -112,6 +123,28 @@ TRTC_CAPI unsigned n_copy_if(void* ptr_in, void* ptr_out, void* ptr_pred); /* ap
 TRTC_CAPI unsigned n_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_pred); /* api.h:154 */
 TRTC_CAPI int n_sort(void* ptr_vec, void* ptr_comp);                  /* api.h:189 */
 TRTC_CAPI int n_sort_by_key(void* ptr_keys, void* ptr_values, void* ptr_comp); /* api.h:190 */
+
+/* ---- scan / compaction consumers (api.h:128-168) ---- */
+TRTC_CAPI unsigned long long n_count(void* ptr_vec, void* ptr_value);  /* api.h:128; (unsigned long long)-1 on error */
+TRTC_CAPI unsigned long long n_count_if(void* ptr_vec, void* ptr_pred); /* api.h:129 */
+TRTC_CAPI unsigned n_reduce_by_key(void* ptr_key_in, void* ptr_value_in, void* ptr_key_out, void* ptr_value_out, void* ptr_binary_pred, void* ptr_binary_op); /* api.h:131 */
+TRTC_CAPI void* n_transform_reduce(void* ptr_vec, void* ptr_unary_op, void* ptr_init, void* ptr_binary_op); /* api.h:137; new DV, caller frees */
+TRTC_CAPI int n_transform_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_unary_op, void* ptr_binary_op); /* api.h:149 */
+TRTC_CAPI int n_transform_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_unary_op, void* ptr_init, void* ptr_binary_op); /* api.h:150; -1 on error (the reference returns 1) */
+TRTC_CAPI unsigned n_remove(void* ptr_vec, void* ptr_value);           /* api.h:155 */
+TRTC_CAPI unsigned n_remove_copy(void* ptr_in, void* ptr_out, void* ptr_value); /* api.h:156 */
+TRTC_CAPI unsigned n_remove_if(void* ptr_vec, void* ptr_pred);         /* api.h:157 */
+TRTC_CAPI unsigned n_remove_copy_if(void* ptr_in, void* ptr_out, void* ptr_pred); /* api.h:158 */
+TRTC_CAPI unsigned n_remove_if_stencil(void* ptr_vec, void* ptr_stencil, void* ptr_pred); /* api.h:159 */
+TRTC_CAPI unsigned n_remove_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_pred); /* api.h:160 */
+TRTC_CAPI unsigned n_unique(void* ptr_vec, void* ptr_binary_pred);    /* api.h:161 */
+TRTC_CAPI unsigned n_unique_copy(void* ptr_vec_in, void* ptr_vec_out, void* ptr_binary_pred); /* api.h:162 */
+TRTC_CAPI unsigned n_unique_by_key(void* ptr_keys, void* ptr_values, void* ptr_binary_pred); /* api.h:163 */
+TRTC_CAPI unsigned n_unique_by_key_copy(void* ptr_keys_in, void* ptr_values_in, void* ptr_key_out, void* ptr_values_out, void* ptr_binary_pred); /* api.h:164 */
+TRTC_CAPI unsigned n_partition(void* ptr_vec, void* ptr_pred);        /* api.h:165 */
+TRTC_CAPI unsigned n_partition_stencil(void* ptr_vec, void* ptr_stencil, void* ptr_pred); /* api.h:166 */
+TRTC_CAPI unsigned n_partition_copy(void* ptr_vec_in, void* ptr_vec_true, void* ptr_vec_false, void* ptr_pred); /* api.h:167 */
+TRTC_CAPI unsigned n_partition_copy_stencil(void* ptr_vec_in, void* ptr_stencil, void* ptr_vec_true, void* ptr_vec_false, void* ptr_pred); /* api.h:168 */
 
 /* ---- extensions: no counterpart in the reference ----
  * Batched segmented primitives (one launch per kernel stage for many

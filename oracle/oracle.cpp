@@ -22,6 +22,16 @@
 //   or_reduce_ref_tree_f32 ....... pinned: equals exact integer sums on integer-
 //                                  valued floats; brute-force tree on tiny n
 //   or_copy_if ................... pinned: test_copy known answers, numpy mask
+//   or_unique / or_reduce_by_key . pinned: the reference tests' inputs
+//                                  (test_unique.py) with their results, groupby
+//                                  restatements (itertools.groupby), per-run
+//                                  numpy reductions
+//   or_select / or_partition ..... pinned: test_remove / test_partition inputs
+//                                  with their results, numpy boolean masks
+//   or_transform_scan ............ pinned: test_transform_scan inputs with their
+//                                  results, numpy cumsum of the transform
+//   or_count ..................... pinned: test_count (20 of 0..1999 have
+//                                  x % 100 == 47), numpy count_nonzero
 //
 // The parallel functions at the bottom (bl_*) are the host baseline that
 // bench.py times: __gnu_parallel / OpenMP on all host cores.
@@ -30,6 +40,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 #include <parallel/algorithm>
 #include <parallel/numeric>
@@ -236,6 +247,75 @@ template <class T> T reduce_seq(const T* in, size_t n, int has_init, T init, int
     default: return -1; \
     }
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// Scan / compaction consumers.  Key equivalence of the restatements below:
+// a == b (EqualTo, built_in.h), or a / div == b / div for integer keys with
+// div > 1 (an equivalence relation, like every binary_pred the tests use).
+// Predicates on one value v (pmode): 0: v % a == b (C++ truncating %),
+// 1: v == a, 2: v != 0 (Identity on an integer stencil).
+// ---------------------------------------------------------------------------
+template <class K> inline bool key_eq(K x, K y, int64_t div)
+{
+    if constexpr (std::is_integral<K>::value) {
+        if (div > 1) return (int64_t)x / div == (int64_t)y / div;
+    }
+    return x == y;
+}
+
+inline bool pred_of(int64_t v, int pmode, int64_t a, int64_t b)
+{
+    if (pmode == 0) return v % a == b;
+    if (pmode == 1) return v == a;
+    return v != 0;
+}
+
+// unique.cpp:5-95: item i is kept when i == 0 or !pred(key[i], key[i-1]);
+// values (vsize bytes each) follow their keys.
+template <class K>
+uint64_t unique_keys(const K* keys, K* kout, const char* vals, char* vout, uint64_t vsize, uint64_t n, int64_t div)
+{
+    uint64_t c = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (i > 0 && key_eq(keys[i], keys[i - 1], div)) continue;
+        kout[c] = keys[i];
+        if (vals) memcpy(vout + c * vsize, vals + i * vsize, vsize);
+        c++;
+    }
+    return c;
+}
+
+// reduce.cpp:36-62: Inclusive_Scan_By_Key of the values (segments start where
+// !pred(key[i-1], key[i])), then the key and scanned value at every segment
+// tail (i == n-1 or !pred(key[i], key[i+1])).
+template <class K, class V>
+uint64_t reduce_by_key(const K* keys, const V* vals, K* kout, V* vout, uint64_t n, int op, int64_t div)
+{
+    uint64_t c = 0;
+    V run = V();
+    for (uint64_t i = 0; i < n; i++) {
+        bool head = i == 0 || !key_eq(keys[i - 1], keys[i], div);
+        run = head ? vals[i] : apply_op<V>(op, run, vals[i]);
+        bool tail = i == n - 1 || !key_eq(keys[i], keys[i + 1], div);
+        if (tail) {
+            kout[c] = keys[i];
+            vout[c] = run;
+            c++;
+        }
+    }
+    return c;
+}
+
+template <class T> inline T unary_of(int u, T x)
+{
+    if (u == 1) return (T)(T(0) - x);  // Negate (wraps for unsigned)
+    if (u == 2) return (T)(x * x);
+    return x;
+}
+
+}  // namespace
+
 extern "C" {
 
 // TRTC_Sort (sort.cpp:7-35, radix_sort.cpp:165-237): in-place stable sort.
@@ -397,6 +477,96 @@ uint64_t or_copy_if_stencil_mod_i32(const int32_t* in, const int32_t* stencil, i
     for (uint64_t i = 0; i < n; i++)
         if (stencil[i] % modulus == remainder) out[k++] = in[i];
     return k;
+}
+
+// Unique / Unique_By_Key (unique.cpp:5-95).  vals may be null.
+uint64_t or_unique(int ktc, const void* keys, void* kout, const void* vals, void* vout, uint64_t vsize, uint64_t n,
+                   int64_t div)
+{
+#define DO(K) return unique_keys<K>((const K*)keys, (K*)kout, (const char*)vals, (char*)vout, vsize, n, div);
+    DISPATCH_TYPE(ktc, DO)
+#undef DO
+}
+
+// Reduce_By_Key (reduce.cpp:36-62), keys ktc, values vtc (value_out type).
+uint64_t or_reduce_by_key(int ktc, int vtc, const void* keys, const void* vals, void* kout, void* vout, uint64_t n,
+                          int op, int64_t div)
+{
+#define DOV(V) \
+    switch (ktc) { \
+    case 4: return reduce_by_key<int32_t, V>((const int32_t*)keys, (const V*)vals, (int32_t*)kout, (V*)vout, n, op, div); \
+    case 5: return reduce_by_key<uint32_t, V>((const uint32_t*)keys, (const V*)vals, (uint32_t*)kout, (V*)vout, n, op, div); \
+    case 6: return reduce_by_key<int64_t, V>((const int64_t*)keys, (const V*)vals, (int64_t*)kout, (V*)vout, n, op, div); \
+    case 8: return reduce_by_key<float, V>((const float*)keys, (const V*)vals, (float*)kout, (V*)vout, n, op, div); \
+    default: return ~0ull; \
+    }
+    DISPATCH_TYPE(vtc, DOV)
+#undef DOV
+}
+
+// Copy_If / Remove family (copy.cpp:16-28, remove.cpp:5-60): stable
+// compaction of in (type tc) keeping items whose predicate (on the int32
+// stencil when given, else on the item) holds, or fails when negate is set
+// (Remove keeps x != value: pmode 1 with negate).
+uint64_t or_select(int tc, const void* in, const int32_t* stencil, void* out, uint64_t n, int pmode, int64_t a,
+                   int64_t b, int negate)
+{
+#define DO(T) { \
+        uint64_t c = 0; \
+        for (uint64_t i = 0; i < n; i++) { \
+            int64_t v = stencil ? (int64_t)stencil[i] : (int64_t)((const T*)in)[i]; \
+            if (pred_of(v, pmode, a, b) != (negate != 0)) ((T*)out)[c++] = ((const T*)in)[i]; \
+        } \
+        return c; }
+    DISPATCH_TYPE(tc, DO)
+#undef DO
+}
+
+// Partition family (partition.cpp:28-130): items whose predicate holds, in
+// order, to tout; the others, in order, to fout.  Returns the first count.
+// The in-place forms are tout = the vector, fout appended after the count.
+uint64_t or_partition(int tc, const void* in, const int32_t* stencil, void* tout, void* fout, uint64_t n, int pmode,
+                      int64_t a, int64_t b)
+{
+#define DO(T) { \
+        uint64_t t = 0, f = 0; \
+        for (uint64_t i = 0; i < n; i++) { \
+            int64_t v = stencil ? (int64_t)stencil[i] : (int64_t)((const T*)in)[i]; \
+            if (pred_of(v, pmode, a, b)) ((T*)tout)[t++] = ((const T*)in)[i]; \
+            else ((T*)fout)[f++] = ((const T*)in)[i]; \
+        } \
+        return t; }
+    DISPATCH_TYPE(tc, DO)
+#undef DO
+}
+
+// Transform_Inclusive/Exclusive_Scan (transform_scan.cpp:4-16): the scan of
+// unary(in[i]) (0 Identity, 1 Negate, 2 square) in the element type;
+// exclusive: init first.
+int or_transform_scan(int tc, const void* in, void* out, uint64_t n, int unary, int op, int exclusive,
+                      const void* init)
+{
+#define DO(T) { \
+        const T* x = (const T*)in; \
+        T* y = (T*)out; \
+        std::vector<T> u(n); \
+        for (uint64_t i = 0; i < n; i++) u[i] = unary_of<T>(unary, x[i]); \
+        if (exclusive) excl_scan<T>(u.data(), y, n, *(const T*)init, op); \
+        else incl_scan<T>(u.data(), y, n, op); \
+        return 0; }
+    DISPATCH_TYPE(tc, DO)
+#undef DO
+}
+
+// Count / Count_If (count.cpp:4-26): number of items whose predicate holds.
+uint64_t or_count(int tc, const void* in, uint64_t n, int pmode, int64_t a, int64_t b)
+{
+#define DO(T) { \
+        uint64_t c = 0; \
+        for (uint64_t i = 0; i < n; i++) c += pred_of((int64_t)((const T*)in)[i], pmode, a, b); \
+        return c; }
+    DISPATCH_TYPE(tc, DO)
+#undef DO
 }
 
 // ---------------------------------------------------------------------------

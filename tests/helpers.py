@@ -71,6 +71,18 @@ class Oracle:
         L.or_copy_if_mod_i32.restype = u64
         L.or_copy_if_stencil_mod_i32.argtypes = [vp, vp, vp, u64, ctypes.c_int32, ctypes.c_int32]
         L.or_copy_if_stencil_mod_i32.restype = u64
+        i64 = ctypes.c_int64
+        L.or_unique.argtypes = [i, vp, vp, vp, vp, u64, u64, i64]
+        L.or_unique.restype = u64
+        L.or_reduce_by_key.argtypes = [i, i, vp, vp, vp, vp, u64, i, i64]
+        L.or_reduce_by_key.restype = u64
+        L.or_select.argtypes = [i, vp, vp, vp, u64, i, i64, i64, i]
+        L.or_select.restype = u64
+        L.or_partition.argtypes = [i, vp, vp, vp, vp, u64, i, i64, i64]
+        L.or_partition.restype = u64
+        L.or_transform_scan.argtypes = [i, vp, vp, u64, i, i, i, vp]
+        L.or_count.argtypes = [i, vp, u64, i, i64, i64]
+        L.or_count.restype = u64
         L.bl_threads.restype = i
         L.bl_sort_pairs_u32.argtypes = [vp, vp, u64]
         L.bl_sort_u32.argtypes = [vp, u64]
@@ -153,6 +165,66 @@ class Oracle:
         out = np.empty_like(a)
         k = self.L.or_copy_if_stencil_mod_i32(a.ctypes.data, s.ctypes.data, out.ctypes.data, len(a), m, r)
         return out[:k]
+
+    # ---- scan / compaction consumers.  Predicates (pmode): ("mod", a, b) is
+    # v % a == b, ("eq", a) is v == a, ("nonzero",) is v != 0.
+    @staticmethod
+    def _pred(p):
+        if p[0] == "mod":
+            return 0, p[1], p[2]
+        if p[0] == "eq":
+            return 1, p[1], 0
+        return 2, 0, 0
+
+    def unique(self, keys, kelem, vals=None, div=1):
+        k = np.ascontiguousarray(np.array(keys, dtype=NP[kelem]))
+        ko = np.empty_like(k)
+        v = None if vals is None else np.ascontiguousarray(np.array(vals))
+        vo = None if v is None else np.empty_like(v)
+        c = self.L.or_unique(TC[kelem], k.ctypes.data, ko.ctypes.data, None if v is None else v.ctypes.data,
+                             None if vo is None else vo.ctypes.data, 0 if v is None else v.itemsize, len(k), div)
+        return (ko[:c],) if v is None else (ko[:c], vo[:c])
+
+    def reduce_by_key(self, keys, vals, kelem, velem, op="Plus", div=1):
+        k = np.ascontiguousarray(np.array(keys, dtype=NP[kelem]))
+        v = np.ascontiguousarray(np.array(vals, dtype=NP[velem]))
+        ko, vo = np.empty_like(k), np.empty_like(v)
+        c = self.L.or_reduce_by_key(TC[kelem], TC[velem], k.ctypes.data, v.ctypes.data, ko.ctypes.data,
+                                    vo.ctypes.data, len(k), OPS[op], div)
+        assert c != 2 ** 64 - 1
+        return ko[:c], vo[:c]
+
+    def select(self, x, elem, pred, stencil=None, negate=False):
+        a = np.ascontiguousarray(np.array(x, dtype=NP[elem]))
+        st = None if stencil is None else np.ascontiguousarray(np.array(stencil, dtype=np.int32))
+        out = np.empty_like(a)
+        m, pa, pb = self._pred(pred)
+        c = self.L.or_select(TC[elem], a.ctypes.data, None if st is None else st.ctypes.data, out.ctypes.data,
+                             len(a), m, pa, pb, int(negate))
+        return out[:c]
+
+    def partition(self, x, elem, pred, stencil=None):
+        a = np.ascontiguousarray(np.array(x, dtype=NP[elem]))
+        st = None if stencil is None else np.ascontiguousarray(np.array(stencil, dtype=np.int32))
+        t, f = np.empty_like(a), np.empty_like(a)
+        m, pa, pb = self._pred(pred)
+        c = self.L.or_partition(TC[elem], a.ctypes.data, None if st is None else st.ctypes.data, t.ctypes.data,
+                                f.ctypes.data, len(a), m, pa, pb)
+        return t[:c], f[:len(a) - c]
+
+    def transform_scan(self, x, elem, unary, op="Plus", exclusive=False, init=0):
+        a = np.ascontiguousarray(np.array(x, dtype=NP[elem]))
+        out = np.empty_like(a)
+        iv = np.array([init], dtype=NP[elem])
+        u = {"Identity": 0, "Negate": 1, "Square": 2}[unary]
+        assert self.L.or_transform_scan(TC[elem], a.ctypes.data, out.ctypes.data, len(a), u, OPS[op],
+                                        int(exclusive), iv.ctypes.data) == 0
+        return out
+
+    def count(self, x, elem, pred):
+        a = np.ascontiguousarray(np.array(x, dtype=NP[elem]))
+        m, pa, pb = self._pred(pred)
+        return self.L.or_count(TC[elem], a.ctypes.data, len(a), m, pa, pb)
 
 
 def gpu_available():

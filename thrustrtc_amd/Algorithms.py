@@ -3,7 +3,7 @@
 Copying.py, Transformations.py)."""
 import ctypes
 import numpy as np
-from .native import native, check_i, check_u, check_cptr
+from .native import native, check_i, check_u, check_ull, check_cptr
 from .DeviceViewable import ValueFromDVPtr
 
 
@@ -54,7 +54,95 @@ def Reduce(vec, value_init=None, binary_op=None):
     return ret
 
 
+def Count(vec, value):
+    return check_ull(native.n_count(vec.m_cptr, value.m_cptr))
+
+
+def Count_If(vec, pred):
+    return check_ull(native.n_count_if(vec.m_cptr, pred.m_cptr))
+
+
+def Transform_Reduce(vec, unary_op, value_init, binary_op):
+    dvptr = check_cptr(native.n_transform_reduce(vec.m_cptr, unary_op.m_cptr, value_init.m_cptr, binary_op.m_cptr))
+    ret = ValueFromDVPtr(dvptr)
+    native.n_dv_destroy(dvptr)
+    return ret
+
+
+def Reduce_By_Key(key_in, value_in, key_out, value_out, binary_pred=None, binary_op=None):
+    return check_u(native.n_reduce_by_key(key_in.m_cptr, value_in.m_cptr, key_out.m_cptr, value_out.m_cptr,
+                                          _p(binary_pred), _p(binary_op)))
+
+
+# ---- PrefixSums.py (transform scans)
+def Transform_Inclusive_Scan(vec_in, vec_out, unary_op, binary_op):
+    check_i(native.n_transform_inclusive_scan(vec_in.m_cptr, vec_out.m_cptr, unary_op.m_cptr, binary_op.m_cptr))
+
+
+def Transform_Exclusive_Scan(vec_in, vec_out, unary_op, value_init, binary_op):
+    check_i(native.n_transform_exclusive_scan(vec_in.m_cptr, vec_out.m_cptr, unary_op.m_cptr, value_init.m_cptr,
+                                              binary_op.m_cptr))
+
+
 # ---- Reordering.py
+def Remove(vec, value):
+    return check_u(native.n_remove(vec.m_cptr, value.m_cptr))
+
+
+def Remove_Copy(vec_in, vec_out, value):
+    return check_u(native.n_remove_copy(vec_in.m_cptr, vec_out.m_cptr, value.m_cptr))
+
+
+def Remove_If(vec, pred):
+    return check_u(native.n_remove_if(vec.m_cptr, pred.m_cptr))
+
+
+def Remove_Copy_If(vec_in, vec_out, pred):
+    return check_u(native.n_remove_copy_if(vec_in.m_cptr, vec_out.m_cptr, pred.m_cptr))
+
+
+def Remove_If_Stencil(vec, stencil, pred):
+    return check_u(native.n_remove_if_stencil(vec.m_cptr, stencil.m_cptr, pred.m_cptr))
+
+
+def Remove_Copy_If_Stencil(vec_in, stencil, vec_out, pred):
+    return check_u(native.n_remove_copy_if_stencil(vec_in.m_cptr, stencil.m_cptr, vec_out.m_cptr, pred.m_cptr))
+
+
+def Unique(vec, binary_pred=None):
+    return check_u(native.n_unique(vec.m_cptr, _p(binary_pred)))
+
+
+def Unique_Copy(vec_in, vec_out, binary_pred=None):
+    return check_u(native.n_unique_copy(vec_in.m_cptr, vec_out.m_cptr, _p(binary_pred)))
+
+
+def Unique_By_Key(keys, values, binary_pred=None):
+    return check_u(native.n_unique_by_key(keys.m_cptr, values.m_cptr, _p(binary_pred)))
+
+
+def Unique_By_Key_Copy(keys_in, values_in, keys_out, values_out, binary_pred=None):
+    return check_u(native.n_unique_by_key_copy(keys_in.m_cptr, values_in.m_cptr, keys_out.m_cptr, values_out.m_cptr,
+                                               _p(binary_pred)))
+
+
+def Partition(vec, pred):
+    return check_u(native.n_partition(vec.m_cptr, pred.m_cptr))
+
+
+def Partition_Stencil(vec, stencil, pred):
+    return check_u(native.n_partition_stencil(vec.m_cptr, stencil.m_cptr, pred.m_cptr))
+
+
+def Partition_Copy(vec_in, vec_true, vec_false, pred):
+    return check_u(native.n_partition_copy(vec_in.m_cptr, vec_true.m_cptr, vec_false.m_cptr, pred.m_cptr))
+
+
+def Partition_Copy_Stencil(vec_in, stencil, vec_true, vec_false, pred):
+    return check_u(native.n_partition_copy_stencil(vec_in.m_cptr, stencil.m_cptr, vec_true.m_cptr, vec_false.m_cptr,
+                                                   pred.m_cptr))
+
+
 def Copy_If(vec_in, vec_out, pred):
     return check_u(native.n_copy_if(vec_in.m_cptr, vec_out.m_cptr, pred.m_cptr))
 

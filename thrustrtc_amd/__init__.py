@@ -17,6 +17,7 @@ from .DeviceViewable import *  # noqa: E402,F401,F403
 from .DVVector import (device_vector, device_vector_from_numpy, device_vector_from_list,  # noqa: E402,F401
                        device_vector_from_dvs, DVVectorLike, DVVector, DVRange, DVVectorAdaptor, DVTorchVector)
 from .Functor import *  # noqa: E402,F401,F403
+from .FakeVectors import *  # noqa: E402,F401,F403
 from .Algorithms import *  # noqa: E402,F401,F403
 
 __version__ = "0.1.0"

@@ -37,10 +37,11 @@ DEVICE_HEADERS = [
     "trtc_scan_by_key.hpp",
     "trtc_radix.hpp",
     "trtc_msort.hpp",
+    "trtc_select.hpp",
     "trtc_kernels.hpp",
 ]
 
-HOST_SOURCES = ["context.cpp", "dvvector.cpp", "algorithms.cpp", "batch.cpp", "segmented.cpp", "server.cpp", "capi.cpp"]
+HOST_SOURCES = ["context.cpp", "dvvector.cpp", "fake_vectors.cpp", "algorithms.cpp", "batch.cpp", "segmented.cpp", "server.cpp", "capi.cpp"]
 
 LIB = os.path.join(PKG, "libPyThrustRTC.so")
 WORKLOADS_LIB = os.path.join(PKG, "libtrtc_workloads.so")

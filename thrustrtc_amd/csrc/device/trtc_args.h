@@ -37,7 +37,7 @@ struct CopyIfArgs
     LookbackArgs lb;
     unsigned long long* count;  // out: total kept (written by the last tile)
     unsigned num_tiles;
-    int has_stencil;            // predicate reads the stencil, not the input
+    int has_stencil;            // informational: kernels take the stencil type (_NoStencil: none)
 };
 
 struct ScanByKeyArgs

@@ -111,10 +111,110 @@ struct RangeView
 {
     typedef typename _TVVALUE::value_t value_t;
     typedef typename _TVVALUE::ref_t ref_t;
-    _TVVALUE _view_vec_value;
+    mutable _TVVALUE _view_vec_value;
     size_t _begin;
     size_t _end;
 
     TRTC_DEV size_t size() const { return _end - _begin; }
     TRTC_DEV ref_t operator[](size_t idx) const { return _view_vec_value[idx + _begin]; }
+};
+
+// Stencil argument of the stencil-less selections (Copy_If, Partition, ...):
+// one byte, never read, so the predicate is only instantiated on the input.
+struct _NoStencil
+{
+    typedef unsigned char value_t;
+    TRTC_DEV value_t operator[](size_t) const { return 0; }
+};
+template <class S> struct _IsNoStencil { static constexpr bool value = false; };
+template <> struct _IsNoStencil<_NoStencil> { static constexpr bool value = true; };
+
+////////// Fake vectors (fake_vectors/*, built_in.h:196-334) ///////////
+// Views computed from an index instead of read from memory.  Member names are
+// the ones the reference's host classes fill in; host layouts follow C rules.
+// operator[] is const (kernels may hold views by const reference); nested
+// views and functors are mutable because their own operators need not be.
+
+// Every element is the same value (DVConstant).  { size_t; T; }
+template <class _T>
+struct ConstantView
+{
+    typedef _T value_t;
+    typedef const _T& ref_t;
+    size_t _size;
+    value_t _value;
+
+    TRTC_DEV size_t size() const { return _size; }
+    TRTC_DEV ref_t operator[](size_t) const { return _value; }
+};
+
+// Element i is init + i (DVCounter).  { size_t; T; }
+template <class _T>
+struct CounterView
+{
+    typedef _T value_t;
+    typedef _T ref_t;
+    size_t _size;
+    value_t _value_init;
+
+    TRTC_DEV size_t size() const { return _size; }
+    TRTC_DEV ref_t operator[](size_t idx) const { return _value_init + (value_t)idx; }
+};
+
+// Assignments to a discard vector's elements go nowhere (DVDiscard).
+template <class _T>
+struct _Sink
+{
+    TRTC_DEV const _Sink& operator=(const _T&) const { return *this; }
+};
+
+// { size_t; _Sink<T>; } -- 16 bytes
+template <class _T>
+struct DiscardView
+{
+    typedef _T value_t;
+    typedef const _Sink<_T>& ref_t;
+    size_t _size;
+    _Sink<_T> _sink;
+
+    TRTC_DEV size_t size() const { return _size; }
+    TRTC_DEV ref_t operator[](size_t) const { return _sink; }
+};
+
+// Element i is value[index[i]] (DVPermutation).  { VV; VI; }
+template <class _TVVALUE, class _TVINDEX>
+struct PermutationView
+{
+    typedef typename _TVVALUE::value_t value_t;
+    typedef typename _TVVALUE::ref_t ref_t;
+    mutable _TVVALUE _view_vec_value;
+    mutable _TVINDEX _view_vec_index;
+
+    TRTC_DEV size_t size() const { return _view_vec_index.size(); }
+    TRTC_DEV ref_t operator[](size_t idx) const { return _view_vec_value[(size_t)_view_vec_index[idx]]; }
+};
+
+// Element i is value[size - 1 - i] (DVReverse).  { VV; }
+template <class _TVVALUE>
+struct ReverseView
+{
+    typedef typename _TVVALUE::value_t value_t;
+    typedef typename _TVVALUE::ref_t ref_t;
+    mutable _TVVALUE _view_vec_value;
+
+    TRTC_DEV size_t size() const { return _view_vec_value.size(); }
+    TRTC_DEV ref_t operator[](size_t idx) const { return _view_vec_value[_view_vec_value.size() - 1 - idx]; }
+};
+
+// Element i is op(in[i]) converted to T (DVTransform).  { VIN; OP; }
+template <class _T, class _T_VIN, class _T_OP>
+struct TransformView
+{
+    typedef _T value_t;
+    typedef _T ref_t;
+    mutable _T_VIN _view_vec_in;
+    mutable _T_OP _view_op;  // user functors have a non-const operator()
+
+    TRTC_DEV size_t size() const { return _view_vec_in.size(); }
+    TRTC_DEV ref_t operator[](size_t idx) const { return (value_t)_view_op(_view_vec_in[idx]); }
 };

@@ -21,6 +21,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     constexpr int WELEMS = 64 * ITEMS;
     constexpr size_t TILE = (size_t)BLOCK * ITEMS;
     typedef typename Sten::value_t ST;
+    constexpr bool HAS_ST = !_IsNoStencil<Sten>::value;
     typedef unsigned long long C;
     constexpr int ESZ = sizeof(T) > sizeof(ST) ? sizeof(T) : sizeof(ST);
     // one transpose buffer per wave, used for the input, then the stencil
@@ -46,7 +47,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     constexpr bool DVS = DirectVec<Sten, ST, ITEMS>::value;
     bool direct_in = false, direct_st = false;
     if constexpr (DVI) direct_in = full && (((size_t)Contig<In>::ptr(in)) & 15) == 0;
-    if constexpr (DVS) direct_st = full && a.has_stencil && (((size_t)Contig<Sten>::ptr(sten)) & 15) == 0;
+    if constexpr (DVS) direct_st = full && HAS_ST && (((size_t)Contig<Sten>::ptr(sten)) & 15) == 0;
     T x[ITEMS];
     int nvalid = ITEMS;
     if constexpr (DVI) {
@@ -69,7 +70,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     }
     unsigned flags = 0;
     unsigned cnt = 0;
-    if (a.has_stencil) {
+    if constexpr (HAS_ST) {
         ST sv[ITEMS];
         if constexpr (DVS) {
             if (direct_st) load_run_vec<ITEMS>(Contig<Sten>::ptr(sten) + wave_base + (size_t)lane * ITEMS, sv);

@@ -13,6 +13,7 @@
 #include "trtc_scan_by_key.hpp"
 #include "trtc_radix.hpp"
 #include "trtc_msort.hpp"
+#include "trtc_select.hpp"
 
 #define TRTC_BLOCK 256
 
@@ -67,6 +68,42 @@ trtc_copy_if_k(In in, Sten sten, Out out, Pred pred, trtc::CopyIfArgs a)
 {
     typedef typename In::value_t T;
     trtc::copy_if_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, out, pred, a);
+}
+
+// Selections on the compaction machinery (Unique, Partition, Reduce_By_Key).
+// Items per thread follow the wider of the key and value types.
+template <class KIn, class KOut, class Pred>
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK) trtc_unique_k(KIn kin, KOut kout, Pred pred, trtc::CopyIfArgs a)
+{
+    typedef typename KIn::value_t K;
+    trtc::unique_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<K>::value, false, K, K>(kin, kin, kout, kout, pred, a);
+}
+
+template <class KIn, class VIn, class KOut, class VOut, class Pred>
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
+trtc_unique_by_key_k(KIn kin, VIn vin, KOut kout, VOut vout, Pred pred, trtc::CopyIfArgs a)
+{
+    typedef typename KIn::value_t K;
+    typedef typename VIn::value_t V;
+    constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::CopyIfItems<K>::value : trtc::CopyIfItems<V>::value;
+    trtc::unique_tile<TRTC_SCAN_BLOCK, ITEMS, true, K, V>(kin, vin, kout, vout, pred, a);
+}
+
+template <class In, class Sten, class TOut, class FOut, class Pred>
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
+trtc_partition_k(In in, Sten sten, TOut tout, FOut fout, Pred pred, trtc::CopyIfArgs a)
+{
+    typedef typename In::value_t T;
+    trtc::partition_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, tout, fout, pred, a);
+}
+
+template <class Keys, class Vals, class KOut, class VOut, class Pred, class Op, class V>
+__global__ void __launch_bounds__(TRTC_BLOCK)
+trtc_reduce_by_key_k(Keys keys, Vals vals, KOut kout, VOut vout, Pred pred, Op op, trtc::CopyIfArgs a)
+{
+    typedef typename Keys::value_t K;
+    constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::ScanByKeyItems<K>::value : trtc::ScanByKeyItems<V>::value;
+    trtc::reduce_by_key_tile<TRTC_BLOCK, ITEMS, V, K>(keys, vals, kout, vout, pred, op, a);
 }
 
 template <class Keys, class Vals, class Out, class Pred, class Op, class V>

@@ -7,7 +7,14 @@
 #include <deque>
 
 #include "copy.h"
+#include "count.h"
+#include "fake_vectors/DVTransform.h"
+#include "partition.h"
 #include "reduce.h"
+#include "remove.h"
+#include "transform_reduce.h"
+#include "transform_scan.h"
+#include "unique.h"
 #include "scan.h"
 #include "sort.h"
 #include "trtc_internal.h"
@@ -251,6 +258,31 @@ bool merge_sort(void* keys, const std::string& T, size_t esize, size_t n, const 
 bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp)
 {
     if (!ensure_init()) return false;
+    if (!contiguous_data(keys) || (values && !contiguous_data(*values))) {
+        // computed views (permutations, reversals, zipped or custom vectors):
+        // sort contiguous copies, then write the results back through the views
+        size_t n = keys.size();
+        if (!keys.is_readable() || !keys.is_writable() ||
+            (values && (!values->is_readable() || !values->is_writable()))) {
+            TRTC_ERR("Sort: vectors must be readable and writable");
+            return false;
+        }
+        if (values && values->size() < n) {
+            TRTC_ERR("Sort_By_Key: values (%zu) shorter than keys (%zu)", values->size(), n);
+            return false;
+        }
+        if (n == 0) return true;
+        hipStream_t s = cur_stream();
+        PoolBuf kb(n * keys.elem_size(), s), vb(values ? n * values->elem_size() : 0, s);
+        if (!kb.p || (values && !vb.p)) return false;
+        DVVectorAdaptor kc(keys.name_elem_cls().c_str(), n, kb.p);
+        DVVectorAdaptor vc(values ? values->name_elem_cls().c_str() : "int32_t", values ? n : 0, vb.p);
+        DVRange vr(values ? *values : keys, 0, n);
+        if (!TRTC_Copy(keys, kc) || (values && !TRTC_Copy(vr, vc))) return false;
+        if (!sort_dispatch(kc, values ? &vc : nullptr, comp)) return false;
+        if (!TRTC_Copy(kc, keys) || (values && !TRTC_Copy(vc, vr))) return false;
+        return check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
     const std::string& cn = comp.name_view_cls();
     int kb = 0, kind = 0;
     if ((cn == "Less" || cn == "Greater") && radix_elem(keys.name_elem_cls(), kb, kind)) {
@@ -397,6 +429,47 @@ bool general_reduce(const DVVectorLike& vec, const DeviceViewable* init, const F
 
 // ---------------------------------------------------------------- copy_if
 
+// The stencil argument of a selection: the stencil's view, or the one-byte
+// _NoStencil marker (the predicate then reads the input).
+std::string stencil_cls(const DVVectorLike* sten) { return sten ? vcls(*sten) : std::string("_NoStencil"); }
+void stencil_arg(Args& args, const DVVectorLike* sten)
+{
+    if (sten) {
+        args.view(*sten);
+    } else {
+        char none = 0;
+        args.raw(&none, 1);
+    }
+}
+
+// Launch a selection kernel (its last argument is CopyIfArgs) over n items in
+// tiles of `tile`, lookback state of `state_bytes`; returns the selected count
+// (synchronous: the count is a host value) or 0xFFFFFFFF.
+uint32_t run_selection(const std::string& name, Args& args, size_t n, size_t tile, unsigned block,
+                       size_t state_bytes, bool has_stencil)
+{
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return (uint32_t)-1;
+    hipStream_t s = cur_stream();
+    size_t tiles = (n + tile - 1) / tile;
+    LookbackScratch lb;
+    if (!lb.make(tiles, state_bytes, s)) return (uint32_t)-1;
+    PoolBuf cnt(8, s);
+    if (!cnt.p) return (uint32_t)-1;
+    trtc::CopyIfArgs a{n, {lb.agg, lb.incl, lb.counter}, cnt.as<unsigned long long>(), (unsigned)tiles,
+                       has_stencil ? 1 : 0};
+    args.val(a);
+    if (!launch_kernel(k, dim3((unsigned)tiles), dim3(block), args.data(), 0, s)) return (uint32_t)-1;
+    unsigned long long count = 0;
+    if (!check_hip(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return (uint32_t)-1;
+    if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return (uint32_t)-1;
+    if (count >= 0xFFFFFFFFull) {
+        TRTC_ERR("selection: count does not fit the 32-bit return value");
+        return (uint32_t)-1;
+    }
+    return (uint32_t)count;
+}
+
 uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVectorLike& vout, const Functor& pred)
 {
     if (!ensure_init()) return (uint32_t)-1;
@@ -406,35 +479,122 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
         TRTC_ERR("Copy_If_Stencil: stencil shorter than input");
         return (uint32_t)-1;
     }
-    const DVVectorLike& st = sten ? *sten : vin;
-    std::string name =
-        "trtc_copy_if_k<" + vcls(vin) + ", " + vcls(st) + ", " + vcls(vout) + ", " + vcls(pred) + ">";
-    KernelRef k;
-    if (!get_template_kernel(name, name, k)) return (uint32_t)-1;
-    hipStream_t s = cur_stream();
-    size_t tile = (size_t)kScanBlock * items_for(vin.elem_size());
-    size_t tiles = (n + tile - 1) / tile;
-    LookbackScratch lb;
-    if (!lb.make(tiles, 8, s)) return (uint32_t)-1;
-    PoolBuf cnt(8, s);
-    if (!cnt.p) return (uint32_t)-1;
-    trtc::CopyIfArgs a{n, {lb.agg, lb.incl, lb.counter}, cnt.as<unsigned long long>(), (unsigned)tiles,
-                       sten ? 1 : 0};
+    std::string name = "trtc_copy_if_k<" + vcls(vin) + ", " + stencil_cls(sten) + ", " + vcls(vout) + ", " +
+                       vcls(pred) + ">";
     Args args;
     args.view(vin);
-    args.view(st);
+    stencil_arg(args, sten);
     args.view(vout);
     args.view(pred);
-    args.val(a);
-    if (!launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s)) return (uint32_t)-1;
-    unsigned long long count = 0;
-    if (!check_hip(hipMemcpyAsync(&count, cnt.p, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return (uint32_t)-1;
-    if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return (uint32_t)-1;
-    if (count >= 0xFFFFFFFFull) {
-        TRTC_ERR("Copy_If: count does not fit the 32-bit return value");
+    return run_selection(name, args, n, (size_t)kScanBlock * items_for(vin.elem_size()), kScanBlock, 8, sten != nullptr);
+}
+
+// Unique / Unique_By_Key (unique.cpp:5-95): keep item i when i == 0 or
+// !pred(key[i], key[i-1]); values, when given, follow their keys.
+uint32_t general_unique(const DVVectorLike& kin, const DVVectorLike* vin, DVVectorLike& kout, DVVectorLike* vout,
+                        const Functor& pred)
+{
+    if (!ensure_init()) return (uint32_t)-1;
+    size_t n = kin.size();
+    if (n == 0) return 0;
+    if (vin && vin->size() < n) {
+        TRTC_ERR("Unique_By_Key: values shorter than keys");
         return (uint32_t)-1;
     }
-    return (uint32_t)count;
+    Args args;
+    std::string name;
+    size_t esz = kin.elem_size();
+    if (vin) {
+        name = "trtc_unique_by_key_k<" + vcls(kin) + ", " + vcls(*vin) + ", " + vcls(kout) + ", " + vcls(*vout) +
+               ", " + vcls(pred) + ">";
+        esz = std::max(esz, vin->elem_size());
+        args.view(kin);
+        args.view(*vin);
+        args.view(kout);
+        args.view(*vout);
+    } else {
+        name = "trtc_unique_k<" + vcls(kin) + ", " + vcls(kout) + ", " + vcls(pred) + ">";
+        args.view(kin);
+        args.view(kout);
+    }
+    args.view(pred);
+    return run_selection(name, args, n, (size_t)kScanBlock * items_for(esz), kScanBlock, 8, false);
+}
+
+// Stable two-way split (partition.cpp:28-130): items whose predicate holds go
+// to tout in order, the others to fout in order; returns the first count.
+uint32_t general_partition_copy(const DVVectorLike& vin, const DVVectorLike* sten, DVVectorLike& tout,
+                                DVVectorLike& fout, const Functor& pred)
+{
+    if (!ensure_init()) return (uint32_t)-1;
+    size_t n = vin.size();
+    if (n == 0) return 0;
+    if (sten && sten->size() < n) {
+        TRTC_ERR("Partition: stencil shorter than input");
+        return (uint32_t)-1;
+    }
+    std::string name = "trtc_partition_k<" + vcls(vin) + ", " + stencil_cls(sten) + ", " + vcls(tout) + ", " +
+                       vcls(fout) + ", " + vcls(pred) + ">";
+    Args args;
+    args.view(vin);
+    stencil_arg(args, sten);
+    args.view(tout);
+    args.view(fout);
+    args.view(pred);
+    return run_selection(name, args, n, (size_t)kScanBlock * items_for(vin.elem_size()), kScanBlock, 8,
+                         sten != nullptr);
+}
+
+// In-place partition: selected items move forward in place (each lands at or
+// before its own index); the rest go to a temporary, then after them.
+uint32_t general_partition(DVVectorLike& vec, const DVVectorLike* sten, const Functor& pred)
+{
+    if (!ensure_init()) return (uint32_t)-1;
+    size_t n = vec.size();
+    if (n == 0) return 0;
+    hipStream_t s = cur_stream();
+    PoolBuf tmp(n * vec.elem_size(), s);
+    if (!tmp.p) return (uint32_t)-1;
+    DVVectorAdaptor falses(vec.name_elem_cls().c_str(), n, tmp.p);
+    uint32_t c = general_partition_copy(vec, sten, vec, falses, pred);
+    if (c == (uint32_t)-1 || c == n) return c;
+    DVVectorAdaptor fpart(falses, 0, n - c);
+    DVRange dst(vec, c, n);
+    if (!TRTC_Copy(fpart, dst)) return (uint32_t)-1;
+    if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return (uint32_t)-1;
+    return c;
+}
+
+// Reduce_By_Key (reduce.cpp:36-62) in one pass: segmented scan of the values
+// and compaction of segment tails (see trtc_select.hpp).
+uint32_t general_reduce_by_key(const DVVectorLike& kin, const DVVectorLike& vin, DVVectorLike& kout,
+                               DVVectorLike& vout, const Functor& pred, const Functor& op)
+{
+    if (!ensure_init()) return (uint32_t)-1;
+    size_t n = kin.size();
+    if (n == 0) return 0;
+    if (vin.size() < n) {
+        TRTC_ERR("Reduce_By_Key: values shorter than keys");
+        return (uint32_t)-1;
+    }
+    if (n >= 0xFFFFFFFFull) {
+        TRTC_ERR("Reduce_By_Key: %zu keys (the count is a 32-bit value)", n);
+        return (uint32_t)-1;
+    }
+    std::string V = canon_type(vout.name_elem_cls());
+    size_t vsz = vout.elem_size();
+    std::string name = "trtc_reduce_by_key_k<" + vcls(kin) + ", " + vcls(vin) + ", " + vcls(kout) + ", " +
+                       vcls(vout) + ", " + vcls(pred) + ", " + vcls(op) + ", " + V + ">";
+    Args args;
+    args.view(kin);
+    args.view(vin);
+    args.view(kout);
+    args.view(vout);
+    args.view(pred);
+    args.view(op);
+    // state {V v; unsigned flag; unsigned tails;} fits align_up(V, 8) + 8 bytes
+    return run_selection(name, args, n, (size_t)kBlock * items_for(std::max(kin.elem_size(), vsz)), kBlock,
+                         align_up(vsz, 8) + 8, false);
 }
 
 // ---------------------------------------------------------------- scan_by_key
@@ -602,4 +762,157 @@ bool TRTC_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out)
                           "    view_vec_out[idx] = (decltype(view_vec_out)::value_t)view_vec_in[idx];\n");
     const DeviceViewable* args[] = {&vec_in, &vec_out};
     return s_for.launch_n(n, args);
+}
+
+// ---------------------------------------------------------------- scan / compaction consumers
+
+// Reduce_By_Key (reduce.cpp:36-62)
+uint32_t TRTC_Reduce_By_Key(const DVVectorLike& key_in, const DVVectorLike& value_in, DVVectorLike& key_out,
+                            DVVectorLike& value_out)
+{
+    return general_reduce_by_key(key_in, value_in, key_out, value_out, Functor("EqualTo"), Functor("Plus"));
+}
+uint32_t TRTC_Reduce_By_Key(const DVVectorLike& key_in, const DVVectorLike& value_in, DVVectorLike& key_out,
+                            DVVectorLike& value_out, const Functor& binary_pred)
+{
+    return general_reduce_by_key(key_in, value_in, key_out, value_out, binary_pred, Functor("Plus"));
+}
+uint32_t TRTC_Reduce_By_Key(const DVVectorLike& key_in, const DVVectorLike& value_in, DVVectorLike& key_out,
+                            DVVectorLike& value_out, const Functor& binary_pred, const Functor& binary_op)
+{
+    return general_reduce_by_key(key_in, value_in, key_out, value_out, binary_pred, binary_op);
+}
+
+// Unique family (unique.cpp:5-95).  The default comparison keeps i when
+// key[i] != key[i-1], which is !EqualTo for every arithmetic type (NaN
+// included).  The in-place forms compact in place.
+uint32_t TRTC_Unique(DVVectorLike& vec) { return general_unique(vec, nullptr, vec, nullptr, Functor("EqualTo")); }
+uint32_t TRTC_Unique(DVVectorLike& vec, const Functor& binary_pred)
+{
+    return general_unique(vec, nullptr, vec, nullptr, binary_pred);
+}
+uint32_t TRTC_Unique_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out)
+{
+    return general_unique(vec_in, nullptr, vec_out, nullptr, Functor("EqualTo"));
+}
+uint32_t TRTC_Unique_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& binary_pred)
+{
+    return general_unique(vec_in, nullptr, vec_out, nullptr, binary_pred);
+}
+uint32_t TRTC_Unique_By_Key(DVVectorLike& keys, DVVectorLike& values)
+{
+    return general_unique(keys, &values, keys, &values, Functor("EqualTo"));
+}
+uint32_t TRTC_Unique_By_Key(DVVectorLike& keys, DVVectorLike& values, const Functor& binary_pred)
+{
+    return general_unique(keys, &values, keys, &values, binary_pred);
+}
+uint32_t TRTC_Unique_By_Key_Copy(const DVVectorLike& keys_in, const DVVectorLike& values_in, DVVectorLike& keys_out,
+                                 DVVectorLike& values_out)
+{
+    return general_unique(keys_in, &values_in, keys_out, &values_out, Functor("EqualTo"));
+}
+uint32_t TRTC_Unique_By_Key_Copy(const DVVectorLike& keys_in, const DVVectorLike& values_in, DVVectorLike& keys_out,
+                                 DVVectorLike& values_out, const Functor& binary_pred)
+{
+    return general_unique(keys_in, &values_in, keys_out, &values_out, binary_pred);
+}
+
+// Partition family (partition.cpp:28-130)
+uint32_t TRTC_Partition(DVVectorLike& vec, const Functor& pred) { return general_partition(vec, nullptr, pred); }
+uint32_t TRTC_Partition_Stencil(DVVectorLike& vec, const DVVectorLike& stencil, const Functor& pred)
+{
+    return general_partition(vec, &stencil, pred);
+}
+uint32_t TRTC_Partition_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_true, DVVectorLike& vec_false,
+                             const Functor& pred)
+{
+    return general_partition_copy(vec_in, nullptr, vec_true, vec_false, pred);
+}
+uint32_t TRTC_Partition_Copy_Stencil(const DVVectorLike& vec_in, const DVVectorLike& stencil, DVVectorLike& vec_true,
+                                     DVVectorLike& vec_false, const Functor& pred)
+{
+    return general_partition_copy(vec_in, &stencil, vec_true, vec_false, pred);
+}
+
+// Remove family (remove.cpp:5-60): compaction of the items to keep (x !=
+// value, or !pred), in place for the in-place forms.
+namespace {
+Functor keep_unequal(const DeviceViewable& value) { return Functor({{"value", &value}}, {"x"}, "        return x != value;\n"); }
+Functor keep_not(const Functor& pred) { return Functor({{"pred", &pred}}, {"x"}, "        return !pred(x);\n"); }
+}  // namespace
+
+uint32_t TRTC_Remove(DVVectorLike& vec, const DeviceViewable& value)
+{
+    return general_copy_if(vec, nullptr, vec, keep_unequal(value));
+}
+uint32_t TRTC_Remove_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out, const DeviceViewable& value)
+{
+    return general_copy_if(vec_in, nullptr, vec_out, keep_unequal(value));
+}
+uint32_t TRTC_Remove_If(DVVectorLike& vec, const Functor& pred)
+{
+    return general_copy_if(vec, nullptr, vec, keep_not(pred));
+}
+uint32_t TRTC_Remove_Copy_If(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& pred)
+{
+    return general_copy_if(vec_in, nullptr, vec_out, keep_not(pred));
+}
+uint32_t TRTC_Remove_If_Stencil(DVVectorLike& vec, const DVVectorLike& stencil, const Functor& pred)
+{
+    return general_copy_if(vec, &stencil, vec, keep_not(pred));
+}
+uint32_t TRTC_Remove_Copy_If_Stencil(const DVVectorLike& vec_in, const DVVectorLike& stencil, DVVectorLike& vec_out,
+                                     const Functor& pred)
+{
+    return general_copy_if(vec_in, &stencil, vec_out, keep_not(pred));
+}
+
+// Transform scans (transform_scan.cpp:4-16): the scans of a TransformView whose
+// elements are unary_op(vec_in[i]) converted to the output element type.
+bool TRTC_Transform_Inclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& unary_op,
+                                   const Functor& binary_op)
+{
+    DVTransform src(vec_in, vec_out.name_elem_cls().c_str(), unary_op);
+    return general_scan(src, vec_out, binary_op, nullptr, false);
+}
+bool TRTC_Transform_Exclusive_Scan(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& unary_op,
+                                   const DeviceViewable& init, const Functor& binary_op)
+{
+    DVTransform src(vec_in, vec_out.name_elem_cls().c_str(), unary_op);
+    return general_scan(src, vec_out, binary_op, &init, true);
+}
+
+// Count / Count_If (count.cpp:4-26): sums of 0/1 indicators in 64 bits.
+bool TRTC_Count(const DVVectorLike& vec, const DeviceViewable& value, size_t& ret)
+{
+    ret = 0;
+    if (vec.size() == 0) return true;
+    Functor ind({{"eq_value", &value}}, {"x"},
+                ("        return (x == (" + vec.name_elem_cls() + ")eq_value) ? (uint64_t)1 : (uint64_t)0;\n").c_str());
+    DVTransform src(vec, "uint64_t", ind);
+    ViewBuf buf;
+    if (!general_reduce(src, nullptr, Functor("Plus"), buf)) return false;
+    memcpy(&ret, buf.data(), sizeof(size_t));
+    return true;
+}
+bool TRTC_Count_If(const DVVectorLike& vec, const Functor& pred, size_t& ret)
+{
+    ret = 0;
+    if (vec.size() == 0) return true;
+    Functor ind({{"pred", &pred}}, {"x"}, "        return pred(x) ? (uint64_t)1 : (uint64_t)0;\n");
+    DVTransform src(vec, "uint64_t", ind);
+    ViewBuf buf;
+    if (!general_reduce(src, nullptr, Functor("Plus"), buf)) return false;
+    memcpy(&ret, buf.data(), sizeof(size_t));
+    return true;
+}
+
+// Transform_Reduce (transform_reduce.cpp:4-15): init op u(v0) op u(v1) ... in
+// the type of init.
+bool TRTC_Transform_Reduce(const DVVectorLike& vec, const Functor& unary_op, const DeviceViewable& init,
+                           const Functor& binary_op, ViewBuf& ret)
+{
+    DVTransform src(vec, init.name_view_cls().c_str(), unary_op);
+    return general_reduce(src, &init, binary_op, ret);
 }

@@ -5,6 +5,20 @@
 #include <cstring>
 
 #include "copy.h"
+#include "count.h"
+#include "partition.h"
+#include "remove.h"
+#include "transform_reduce.h"
+#include "transform_scan.h"
+#include "unique.h"
+#include "fake_vectors/DVConstant.h"
+#include "fake_vectors/DVCounter.h"
+#include "fake_vectors/DVCustomVector.h"
+#include "fake_vectors/DVDiscard.h"
+#include "fake_vectors/DVPermutation.h"
+#include "fake_vectors/DVReverse.h"
+#include "fake_vectors/DVTransform.h"
+#include "fake_vectors/DVZipped.h"
 #include "reduce.h"
 #include "scan.h"
 #include "sort.h"
@@ -218,6 +232,39 @@ void* n_dvrange_create(void* ptr_in, unsigned long long begin, unsigned long lon
     return new DVRange(*v, begin, end);
 }
 
+// ---- fake vectors
+void* n_dvconstant_create(void* cptr, unsigned long long size) { return new DVConstant(*(DeviceViewable*)cptr, size); }
+void* n_dvcounter_create(void* cptr, unsigned long long size) { return new DVCounter(*(DeviceViewable*)cptr, size); }
+void* n_dvdiscard_create(const char* elem_cls, unsigned long long size) { return new DVDiscard(elem_cls, size); }
+void* n_dvpermutation_create(void* ptr_value, void* ptr_index)
+{
+    return new DVPermutation(*(DVVectorLike*)ptr_value, *(DVVectorLike*)ptr_index);
+}
+void* n_dvreverse_create(void* ptr_value) { return new DVReverse(*(DVVectorLike*)ptr_value); }
+void* n_dvtransform_create(void* ptr_in, const char* elem_cls, void* ptr_op)
+{
+    return new DVTransform(*(DVVectorLike*)ptr_in, elem_cls, *(Functor*)ptr_op);
+}
+void* n_dvzipped_create(void* ptr_vecs, void* ptr_elem_names)
+{
+    const PtrArray& v = *(PtrArray*)ptr_vecs;
+    const StrArray& names = *(StrArray*)ptr_elem_names;
+    if (v.empty() || v.size() != names.size()) return nullptr;
+    std::vector<DVVectorLike*> vecs;
+    for (auto* p : v) vecs.push_back((DVVectorLike*)p);
+    return new DVZipped(vecs, cstrs(names));
+}
+void* n_dvcustomvector_create(void* ptr_dvs, void* ptr_names, const char* name_idx, const char* body,
+                              const char* elem_cls, unsigned long long size, unsigned read_only)
+{
+    const PtrArray& dvs = *(PtrArray*)ptr_dvs;
+    const StrArray& names = *(StrArray*)ptr_names;
+    if (dvs.size() != names.size()) return nullptr;
+    std::vector<CapturedDeviceViewable> arg_map(dvs.size());
+    for (size_t i = 0; i < dvs.size(); i++) arg_map[i] = {names[i].c_str(), dvs[i]};
+    return new DVCustomVector(arg_map, name_idx, body, elem_cls, size, read_only != 0);
+}
+
 void* n_dvvectorlike_data(void* cptr) { return contiguous_data(*(DVVectorLike*)cptr); }
 
 // ---- functors
@@ -330,6 +377,124 @@ unsigned n_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void*
 {
     return TRTC_Copy_If_Stencil(*(DVVectorLike*)ptr_in, *(DVVectorLike*)ptr_stencil, *(DVVectorLike*)ptr_out,
                                 *(Functor*)ptr_pred);
+}
+
+// ---- scan / compaction consumers (reference python/api_Reductions.cpp,
+// api_PrefixSums.cpp, api_Reordering.cpp): a NULL functor selects the
+// default overload
+typedef DVVectorLike DVL;
+
+unsigned long long n_count(void* ptr_vec, void* ptr_value)
+{
+    size_t r;
+    return TRTC_Count(*(DVL*)ptr_vec, *(DeviceViewable*)ptr_value, r) ? (unsigned long long)r : ~0ull;
+}
+
+unsigned long long n_count_if(void* ptr_vec, void* ptr_pred)
+{
+    size_t r;
+    return TRTC_Count_If(*(DVL*)ptr_vec, *(Functor*)ptr_pred, r) ? (unsigned long long)r : ~0ull;
+}
+
+unsigned n_reduce_by_key(void* ptr_key_in, void* ptr_value_in, void* ptr_key_out, void* ptr_value_out,
+                         void* ptr_binary_pred, void* ptr_binary_op)
+{
+    DVL& ki = *(DVL*)ptr_key_in;
+    DVL& vi = *(DVL*)ptr_value_in;
+    DVL& ko = *(DVL*)ptr_key_out;
+    DVL& vo = *(DVL*)ptr_value_out;
+    if (!ptr_binary_pred) return TRTC_Reduce_By_Key(ki, vi, ko, vo);
+    if (!ptr_binary_op) return TRTC_Reduce_By_Key(ki, vi, ko, vo, *(Functor*)ptr_binary_pred);
+    return TRTC_Reduce_By_Key(ki, vi, ko, vo, *(Functor*)ptr_binary_pred, *(Functor*)ptr_binary_op);
+}
+
+void* n_transform_reduce(void* ptr_vec, void* ptr_unary_op, void* ptr_init, void* ptr_binary_op)
+{
+    DeviceViewable* init = (DeviceViewable*)ptr_init;
+    ViewBuf ret;
+    if (!TRTC_Transform_Reduce(*(DVL*)ptr_vec, *(Functor*)ptr_unary_op, *init, *(Functor*)ptr_binary_op, ret))
+        return nullptr;
+    return dv_from_viewbuf(ret, init->name_view_cls().c_str());
+}
+
+int n_transform_inclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_unary_op, void* ptr_binary_op)
+{
+    return TRTC_Transform_Inclusive_Scan(*(DVL*)ptr_vec_in, *(DVL*)ptr_vec_out, *(Functor*)ptr_unary_op,
+                                         *(Functor*)ptr_binary_op)
+               ? 0
+               : -1;
+}
+
+int n_transform_exclusive_scan(void* ptr_vec_in, void* ptr_vec_out, void* ptr_unary_op, void* ptr_init,
+                               void* ptr_binary_op)
+{
+    return TRTC_Transform_Exclusive_Scan(*(DVL*)ptr_vec_in, *(DVL*)ptr_vec_out, *(Functor*)ptr_unary_op,
+                                         *(DeviceViewable*)ptr_init, *(Functor*)ptr_binary_op)
+               ? 0
+               : -1;
+}
+
+unsigned n_remove(void* ptr_vec, void* ptr_value) { return TRTC_Remove(*(DVL*)ptr_vec, *(DeviceViewable*)ptr_value); }
+unsigned n_remove_copy(void* ptr_in, void* ptr_out, void* ptr_value)
+{
+    return TRTC_Remove_Copy(*(DVL*)ptr_in, *(DVL*)ptr_out, *(DeviceViewable*)ptr_value);
+}
+unsigned n_remove_if(void* ptr_vec, void* ptr_pred) { return TRTC_Remove_If(*(DVL*)ptr_vec, *(Functor*)ptr_pred); }
+unsigned n_remove_copy_if(void* ptr_in, void* ptr_out, void* ptr_pred)
+{
+    return TRTC_Remove_Copy_If(*(DVL*)ptr_in, *(DVL*)ptr_out, *(Functor*)ptr_pred);
+}
+unsigned n_remove_if_stencil(void* ptr_vec, void* ptr_stencil, void* ptr_pred)
+{
+    return TRTC_Remove_If_Stencil(*(DVL*)ptr_vec, *(DVL*)ptr_stencil, *(Functor*)ptr_pred);
+}
+unsigned n_remove_copy_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_pred)
+{
+    return TRTC_Remove_Copy_If_Stencil(*(DVL*)ptr_in, *(DVL*)ptr_stencil, *(DVL*)ptr_out, *(Functor*)ptr_pred);
+}
+
+unsigned n_unique(void* ptr_vec, void* ptr_binary_pred)
+{
+    DVL& v = *(DVL*)ptr_vec;
+    return ptr_binary_pred ? TRTC_Unique(v, *(Functor*)ptr_binary_pred) : TRTC_Unique(v);
+}
+unsigned n_unique_copy(void* ptr_vec_in, void* ptr_vec_out, void* ptr_binary_pred)
+{
+    DVL& i = *(DVL*)ptr_vec_in;
+    DVL& o = *(DVL*)ptr_vec_out;
+    return ptr_binary_pred ? TRTC_Unique_Copy(i, o, *(Functor*)ptr_binary_pred) : TRTC_Unique_Copy(i, o);
+}
+unsigned n_unique_by_key(void* ptr_keys, void* ptr_values, void* ptr_binary_pred)
+{
+    DVL& k = *(DVL*)ptr_keys;
+    DVL& v = *(DVL*)ptr_values;
+    return ptr_binary_pred ? TRTC_Unique_By_Key(k, v, *(Functor*)ptr_binary_pred) : TRTC_Unique_By_Key(k, v);
+}
+unsigned n_unique_by_key_copy(void* ptr_keys_in, void* ptr_values_in, void* ptr_key_out, void* ptr_values_out,
+                              void* ptr_binary_pred)
+{
+    DVL& ki = *(DVL*)ptr_keys_in;
+    DVL& vi = *(DVL*)ptr_values_in;
+    DVL& ko = *(DVL*)ptr_key_out;
+    DVL& vo = *(DVL*)ptr_values_out;
+    return ptr_binary_pred ? TRTC_Unique_By_Key_Copy(ki, vi, ko, vo, *(Functor*)ptr_binary_pred)
+                           : TRTC_Unique_By_Key_Copy(ki, vi, ko, vo);
+}
+
+unsigned n_partition(void* ptr_vec, void* ptr_pred) { return TRTC_Partition(*(DVL*)ptr_vec, *(Functor*)ptr_pred); }
+unsigned n_partition_stencil(void* ptr_vec, void* ptr_stencil, void* ptr_pred)
+{
+    return TRTC_Partition_Stencil(*(DVL*)ptr_vec, *(DVL*)ptr_stencil, *(Functor*)ptr_pred);
+}
+unsigned n_partition_copy(void* ptr_vec_in, void* ptr_vec_true, void* ptr_vec_false, void* ptr_pred)
+{
+    return TRTC_Partition_Copy(*(DVL*)ptr_vec_in, *(DVL*)ptr_vec_true, *(DVL*)ptr_vec_false, *(Functor*)ptr_pred);
+}
+unsigned n_partition_copy_stencil(void* ptr_vec_in, void* ptr_stencil, void* ptr_vec_true, void* ptr_vec_false,
+                                  void* ptr_pred)
+{
+    return TRTC_Partition_Copy_Stencil(*(DVL*)ptr_vec_in, *(DVL*)ptr_stencil, *(DVL*)ptr_vec_true,
+                                       *(DVL*)ptr_vec_false, *(Functor*)ptr_pred);
 }
 
 int n_sort(void* ptr_vec, void* ptr_comp)

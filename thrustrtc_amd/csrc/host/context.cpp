@@ -833,6 +833,7 @@ void* contiguous_data(const DVVectorLike& v)
 {
     if (auto p = dynamic_cast<const DVVector*>(&v)) return p->data();
     if (auto p = dynamic_cast<const DVVectorAdaptor*>(&v)) return p->data();
+    if (auto p = dynamic_cast<const DVRange*>(&v)) return p->contiguous();
     return nullptr;
 }
 
@@ -912,6 +913,12 @@ size_t TRTC_Size_Of(const char* cls)
     if (si) return si->size;
     std::string c = cls;
     if (c.compare(0, 11, "VectorView<") == 0) return 16;
+    {
+        // structs whose layout the host computed (functors, views, zipped elements)
+        std::lock_guard<std::mutex> lk(g_layout_mtx);
+        auto it = g_struct_layout.find(c);
+        if (it != g_struct_layout.end()) return it->second.first;
+    }
     static std::mutex m;
     static std::unordered_map<std::string, size_t> cache;
     {

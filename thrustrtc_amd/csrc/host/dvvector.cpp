@@ -112,6 +112,8 @@ DVRange::DVRange(const DVVectorLike& vec_value, size_t begin, size_t end)
     m_begin = begin;
     m_end = end;
     m_name_view_cls = std::string("RangeView<") + vec_value.name_view_cls() + ">";
+    void* base = contiguous_data(vec_value);
+    m_contig = base ? (char*)base + begin * vec_value.elem_size() : nullptr;
 }
 
 ViewBuf DVRange::view() const

@@ -156,7 +156,7 @@ bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, voi
                      int kind, bool descending, int val_bytes, hipStream_t s);
 
 // ---------------------------------------------------------------- vectors
-// Raw device pointer of a contiguous vector-like (DVVector, DVVectorAdaptor),
+// Raw device pointer of a contiguous vector-like (DVVector, DVVectorAdaptor, a DVRange of one),
 // nullptr otherwise.
 void* contiguous_data(const DVVectorLike& v);
 

@@ -74,6 +74,34 @@ SIGNATURES = {
     "n_exclusive_scan": (_i, [_vp, _vp, _vp, _vp]),
     "n_inclusive_scan_by_key": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "n_exclusive_scan_by_key": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "n_count": (_ull, [_vp, _vp]),
+    "n_count_if": (_ull, [_vp, _vp]),
+    "n_reduce_by_key": (_u, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "n_transform_reduce": (_vp, [_vp, _vp, _vp, _vp]),
+    "n_transform_inclusive_scan": (_i, [_vp, _vp, _vp, _vp]),
+    "n_transform_exclusive_scan": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "n_remove": (_u, [_vp, _vp]),
+    "n_remove_copy": (_u, [_vp, _vp, _vp]),
+    "n_remove_if": (_u, [_vp, _vp]),
+    "n_remove_copy_if": (_u, [_vp, _vp, _vp]),
+    "n_remove_if_stencil": (_u, [_vp, _vp, _vp]),
+    "n_remove_copy_if_stencil": (_u, [_vp, _vp, _vp, _vp]),
+    "n_unique": (_u, [_vp, _vp]),
+    "n_unique_copy": (_u, [_vp, _vp, _vp]),
+    "n_unique_by_key": (_u, [_vp, _vp, _vp]),
+    "n_unique_by_key_copy": (_u, [_vp, _vp, _vp, _vp, _vp]),
+    "n_partition": (_u, [_vp, _vp]),
+    "n_partition_stencil": (_u, [_vp, _vp, _vp]),
+    "n_partition_copy": (_u, [_vp, _vp, _vp, _vp]),
+    "n_partition_copy_stencil": (_u, [_vp, _vp, _vp, _vp, _vp]),
+    "n_dvconstant_create": (_vp, [_vp, _ull]),
+    "n_dvcounter_create": (_vp, [_vp, _ull]),
+    "n_dvdiscard_create": (_vp, [_s, _ull]),
+    "n_dvpermutation_create": (_vp, [_vp, _vp]),
+    "n_dvreverse_create": (_vp, [_vp]),
+    "n_dvtransform_create": (_vp, [_vp, _s, _vp]),
+    "n_dvzipped_create": (_vp, [_vp, _vp]),
+    "n_dvcustomvector_create": (_vp, [_vp, _vp, _s, _s, _s, _ull, _u]),
     "n_copy_if": (_u, [_vp, _vp, _vp]),
     "n_copy_if_stencil": (_u, [_vp, _vp, _vp, _vp]),
     "n_sort": (_i, [_vp, _vp]),
