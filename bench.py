@@ -168,6 +168,119 @@ def leg_single_pass(trtc, native, wl, a, log):
     return res
 
 
+def leg_consumers(trtc, native, wl, a, log):
+    """The scan / compaction consumers and the comparison sort on 2^28 int32
+    items: fused Reduce_By_Key and Unique_Copy over keys in runs of 1..64
+    (log-uniform), Partition_Copy / Remove_Copy_If with x % 3 == 0,
+    Transform_Inclusive_Scan (Negate), and Sort with a user comparator (merge
+    path) on 2^24 uint32 keys.  Each GPU result is compared with the oracle's
+    (which is also the timed single-core host baseline)."""
+    import numpy as np
+    n = 1 << 28
+    reps = 5
+    res = {}
+    sizes, _ = wl.segment_sizes(n // 8, 51, 1, 64)
+    k = int(np.searchsorted(np.cumsum(sizes.astype(np.int64)), n)) + 1
+    keys = np.repeat(np.arange(k, dtype=np.int32), sizes[:k].astype(np.int64))[:n]
+    vals = wl.i32_range(n, 52, -1000, 1000)
+    x = wl.i32_range(n, 5, -1000, 1000)
+    dk, dv, dx = (trtc.device_vector_from_numpy(v) for v in (keys, vals, x))
+    o1, o2 = trtc.device_vector("int32_t", n), trtc.device_vector("int32_t", n)
+    mod3 = trtc.Functor({}, ["x"], "        return x % 3 == 0;\n")
+    orc = None
+    if not a.no_cpu:
+        from helpers import Oracle
+        orc = Oracle()
+
+    def entry(name, fn, prefix, algo_fn, check):
+        fn()
+        trtc.Wait()
+        out = [None]
+
+        def call():
+            out[0] = fn()
+        dt, prof = profiled(native, call, reps)
+        kname, kms = kernel_ms(prof, prefix)
+        algo = algo_fn(out[0])
+        r = {"ms": round(dt * 1e3, 4), "kernel": kname, "kernel_ms": round(kms, 4), "algorithmic_bytes": algo,
+             "kernel_gbs": round(algo / kms / 1e6, 1), "frac_hbm": round(algo / kms / 1e6 / HBM_PEAK_GBS, 4)}
+        if orc is not None:
+            t0 = time.perf_counter()
+            ok = check(out[0])
+            r["cpu_oracle_s"] = round(time.perf_counter() - t0, 3)
+            r["cpu_gbs"] = round(algo / r["cpu_oracle_s"] / 1e9, 3)
+            r["bit_exact"] = bool(ok)
+        res[name] = r
+        log(f"{name}: {r}")
+
+    def chk_rbk(c):
+        wk, wvals = orc.reduce_by_key(keys, vals, "int32_t", "int32_t")
+        return c == len(wk) and (o1.to_host(0, c) == wk).all() and (o2.to_host(0, c) == wvals).all()
+    entry("reduce_by_key_i32", lambda: trtc.Reduce_By_Key(dk, dv, o1, o2), "trtc_reduce_by_key_k<",
+          lambda c: 8 * n + 8 * c, chk_rbk)
+
+    def chk_uniq(c):
+        (w,) = orc.unique(keys, "int32_t")
+        return c == len(w) and (o1.to_host(0, c) == w).all()
+    entry("unique_copy_i32", lambda: trtc.Unique_Copy(dk, o1), "trtc_unique_k<", lambda c: 4 * n + 4 * c, chk_uniq)
+    del dk, dv
+
+    def chk_part(c):
+        wt, wf = orc.partition(x, "int32_t", ("mod", 3, 0))
+        return c == len(wt) and (o1.to_host(0, c) == wt).all() and (o2.to_host(0, n - c) == wf).all()
+    entry("partition_copy_i32_mod3", lambda: trtc.Partition_Copy(dx, o1, o2, mod3), "trtc_partition_k<",
+          lambda c: 8 * n, chk_part)
+
+    def chk_rem(c):
+        w = orc.select(x, "int32_t", ("mod", 3, 0), negate=True)
+        return c == len(w) and (o1.to_host(0, c) == w).all()
+    entry("remove_copy_if_i32_mod3", lambda: trtc.Remove_Copy_If(dx, o1, mod3), "trtc_copy_if_k<",
+          lambda c: 4 * n + 4 * c, chk_rem)
+
+    def chk_tscan(_):
+        return (o1.to_host() == orc.transform_scan(x, "int32_t", "Negate")).all()
+    entry("transform_inclusive_scan_i32_negate", lambda: trtc.Transform_Inclusive_Scan(dx, o1, trtc.Negate(),
+          trtc.Plus()), "trtc_scan_k<", lambda _: 8 * n, chk_tscan)
+    del dx, o1, o2
+
+    # comparison sort with a user comparator: LDS tile sort + merge passes
+    ns = 1 << 24
+    ks = wl.u32_uniform(ns, 2)
+    src = trtc.device_vector_from_numpy(ks)
+    work = trtc.device_vector("uint32_t", ns)
+    less = trtc.Functor({}, ["x", "y"], "        return x < y;\n")
+    trtc.Copy(src, work)
+    trtc.Sort(work, less)
+    trtc.Wait()
+
+    def msort():
+        trtc.Copy(src, work)
+        trtc.Sort(work, less)
+    dt, prof = profiled(native, msort, reps)
+    mk = {kk: v for kk, v in prof.items() if kk.startswith("trtc_msort")}
+    kms = sum(v["total_ms"] for v in mk.values()) / reps
+    passes = 1 + max(0, int(np.ceil(np.log2(ns / 2048))))
+    algo = passes * 8 * ns  # every pass reads and writes each 4-byte key once
+    r = {"n": ns, "ms_incl_restore": round(dt * 1e3, 4), "kernels_ms": round(kms, 4), "passes": passes,
+         "mkeys_per_s": round(ns / kms / 1e3, 1), "algorithmic_bytes": algo,
+         "kernel_gbs": round(algo / kms / 1e6, 1), "frac_hbm": round(algo / kms / 1e6 / HBM_PEAK_GBS, 4),
+         "kernels": mk}
+    if orc is not None:
+        got = work.to_host()
+        ck = ks.copy()
+        t0 = time.perf_counter()
+        orc.L.bl_sort_u32(ck.ctypes.data, ns)
+        r["cpu_sort_s"] = round(time.perf_counter() - t0, 3)
+        r["cpu_cores"] = int(orc.L.bl_threads())
+        r["bit_exact"] = bool((got == ck).all())
+    res["merge_sort_u32_user_comparator_2p24"] = r
+    log(f"merge sort: {r}")
+    res["cpu"] = {"kind": "port", "cores": 1,
+                  "sample": "the full inputs, one sequential oracle call each (the same call checks the GPU "
+                            "result); the merge-sort row uses __gnu_parallel::sort on all cores"}
+    return res
+
+
 def leg_small_calls(trtc, native, wl, log, cpu=True):
     """Latency of single small calls: TRTC_Sort on 2^20 uint32 (seed 0) and
     TRTC_Inclusive_Scan on 2^20 int32 in [-100, 100] (seed 0); median over
@@ -418,7 +531,9 @@ def main():
     ap.add_argument("--segments", type=int, default=4096, help="segments of the batched-segments leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--served-requests", type=int, default=3000, help="requests per served-stream run")
-    ap.add_argument("--only", choices=["sort", "single_pass", "small", "segmented", "served"], default=None,
+    ap.add_argument("--no-consumers", action="store_true", help="skip the scan / compaction consumers leg")
+    ap.add_argument("--only", choices=["sort", "single_pass", "small", "segmented", "served", "consumers"],
+                    default=None,
                     help="run one leg only (the headline fields are then absent)")
     ap.add_argument("--mode", choices=["sort", "copy"], default="sort",
                     help="copy: a plain device copy of a known size (4 B/elem read + 4 B/elem write), "
@@ -452,6 +567,8 @@ def main():
             res = leg_small_calls(trtc, native, wl, log, not a.no_cpu)
         elif a.only == "served":
             res = leg_served(trtc, native, wl, a, log)
+        elif a.only == "consumers":
+            res = leg_consumers(trtc, native, wl, a, log)
         else:
             res = leg_segmented(trtc, native, wl, a, log)
         print(json.dumps({"leg": a.only, "result": res}), flush=True)
@@ -573,6 +690,7 @@ def main():
     served = None if a.no_served else leg_served(trtc, native, wl, a, log)
     single_pass = None if a.no_single_pass else leg_single_pass(trtc, native, wl, a, log)
     small = None if a.no_small else leg_small_calls(trtc, native, wl, log, not a.no_cpu)
+    consumers = None if a.no_consumers else leg_consumers(trtc, native, wl, a, log)
     copy_bw = leg_copy(trtc, N, 5, 2)
     log(f"copy bandwidth: {copy_bw}")
 
@@ -605,6 +723,7 @@ def main():
         "batched_segments": segmented,
         "served_stream": served,
         "small_calls_2p20": small,
+        "consumers_2p28": consumers,
         "device_copy": copy_bw,
     }
     print(json.dumps(line), flush=True)
