@@ -170,7 +170,7 @@ def leg_single_pass(trtc, native, wl, a, log):
 
 def leg_consumers(trtc, native, wl, a, log):
     """The scan / compaction consumers and the comparison sort on 2^28 int32
-    items: fused Reduce_By_Key and Unique_Copy over keys in runs of 1..64
+    items: fused Reduce_By_Key, Inclusive_Scan_By_Key and Unique_Copy over keys in runs of 1..64
     (log-uniform), Partition_Copy / Remove_Copy_If with x % 3 == 0,
     Transform_Inclusive_Scan (Negate), and Sort with a user comparator (merge
     path) on 2^24 uint32 keys.  Each GPU result is compared with the oracle's
@@ -218,6 +218,11 @@ def leg_consumers(trtc, native, wl, a, log):
         return c == len(wk) and (o1.to_host(0, c) == wk).all() and (o2.to_host(0, c) == wvals).all()
     entry("reduce_by_key_i32", lambda: trtc.Reduce_By_Key(dk, dv, o1, o2), "trtc_reduce_by_key_k<",
           lambda c: 8 * n + 8 * c, chk_rbk)
+
+    def chk_sbk(_):
+        return (o1.to_host() == orc.scan_by_key(keys, vals, "int32_t", "int32_t")).all()
+    entry("inclusive_scan_by_key_i32", lambda: trtc.Inclusive_Scan_By_Key(dk, dv, o1), "trtc_scan_by_key_k<",
+          lambda _: 12 * n, chk_sbk)
 
     def chk_uniq(c):
         (w,) = orc.unique(keys, "int32_t")

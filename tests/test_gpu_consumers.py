@@ -20,7 +20,7 @@ from helpers import Oracle, workloads
 pytestmark = pytest.mark.gpu
 
 KA = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "consumer_known_answers.json")))
-SIZES = [1, 7, 8191, 8192, 8193, 100003, 1 << 20]
+SIZES = [1, 7, 64, 1025, 8191, 8192, 8193, 100003, 1 << 20]  # 7, 64, 1025: waves past n
 SENT = -7777
 
 

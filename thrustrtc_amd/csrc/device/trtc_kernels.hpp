@@ -97,21 +97,24 @@ trtc_partition_k(In in, Sten sten, TOut tout, FOut fout, Pred pred, trtc::CopyIf
     trtc::partition_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, tout, fout, pred, a);
 }
 
+// The by-key kernels use 512-thread tiles as well (2^28 int32: Reduce_By_Key
+// 1.23 -> 0.95 ms, Scan_By_Key 1.45 -> 1.37 ms against 256-thread tiles).
 template <class Keys, class Vals, class KOut, class VOut, class Pred, class Op, class V>
-__global__ void __launch_bounds__(TRTC_BLOCK)
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_reduce_by_key_k(Keys keys, Vals vals, KOut kout, VOut vout, Pred pred, Op op, trtc::CopyIfArgs a)
 {
     typedef typename Keys::value_t K;
     constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::ScanByKeyItems<K>::value : trtc::ScanByKeyItems<V>::value;
-    trtc::reduce_by_key_tile<TRTC_BLOCK, ITEMS, V, K>(keys, vals, kout, vout, pred, op, a);
+    trtc::reduce_by_key_tile<TRTC_SCAN_BLOCK, ITEMS, V, K>(keys, vals, kout, vout, pred, op, a);
 }
 
 template <class Keys, class Vals, class Out, class Pred, class Op, class V>
-__global__ void __launch_bounds__(TRTC_BLOCK)
+__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_scan_by_key_k(Keys keys, Vals vals, Out out, Pred pred, Op op, V init, trtc::ScanByKeyArgs a)
 {
     typedef typename Keys::value_t K;
-    trtc::scan_by_key_tile<TRTC_BLOCK, trtc::ScanByKeyItems<V>::value, V, K>(keys, vals, out, pred, op, init, a);
+    constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::ScanByKeyItems<K>::value : trtc::ScanByKeyItems<V>::value;
+    trtc::scan_by_key_tile<TRTC_SCAN_BLOCK, ITEMS, V, K>(keys, vals, out, pred, op, init, a);
 }
 
 template <int KB>

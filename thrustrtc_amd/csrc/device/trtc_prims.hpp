@@ -283,4 +283,37 @@ template <int ITEMS> struct WaveTile
     TRTC_DEV static int pad(int i) { return i + (i >> 5); }
 };
 
+// Valid items of this lane in a blocked layout (wave_base + lane*ITEMS + j).
+template <int ITEMS> TRTC_DEV int lane_valid(size_t wave_base, size_t n, bool full)
+{
+    if (full) return ITEMS;
+    size_t first = wave_base + (size_t)lane_id() * ITEMS;
+    return first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
+}
+
+// Blocked load of a wave's 64*ITEMS items: 16-byte vector loads of each
+// lane's run for contiguous views of 4/8-byte elements, else coalesced
+// striped loads transposed through the wave's LDS buffer `lds`.
+template <int ITEMS, class T, class View>
+TRTC_DEV void load_blocked(View& v, size_t wave_base, size_t n, bool full, int nvalid, T* lds, T (&x)[ITEMS])
+{
+    const unsigned lane = lane_id();
+    if constexpr (DirectVec<View, T, ITEMS>::value) {
+        if (full && (((size_t)Contig<View>::ptr(v)) & 15) == 0) {
+            load_run_vec<ITEMS>(Contig<View>::ptr(v) + wave_base + (size_t)lane * ITEMS, x);
+            return;
+        }
+    }
+    wave_sync();  // the buffer's previous contents are consumed
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        int i = k * 64 + (int)lane;
+        if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)v[wave_base + i];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++)
+        if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
+}
+
 }  // namespace trtc

@@ -63,36 +63,22 @@ TRTC_DEV void scan_by_key_tile(Keys& keys, Vals& vals, Out& out, Pred& pred, Op&
     K* ldk = s_k[wave];
     SegOp<V, Op> sop{op};
 
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        int i = k * 64 + (int)lane;
-        if (full || wave_base + i < n) {
-            ldv[WaveTile<ITEMS>::pad(i)] = (V)vals[wave_base + i];
-            ldk[WaveTile<ITEMS>::pad(i)] = (K)keys[wave_base + i];
-        }
-    }
-    // key just before this wave's range (keys are never written by the scan)
-    K kprev_wave;
-    if (wave_base > 0) kprev_wave = (K)keys[wave_base - 1];
-    wave_sync();
-
-    int nvalid = ITEMS;
-    if (!full) {
-        size_t first = wave_base + (size_t)lane * ITEMS;
-        nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
-    }
+    // blocked loads (16-byte vector loads for contiguous views); the key
+    // before this lane's first item is the previous lane's last, or for lane
+    // 0 the item before the wave (keys are never written by the scan)
+    const int nvalid = lane_valid<ITEMS>(wave_base, n, full);
+    K kx[ITEMS];
     V x[ITEMS];
+    load_blocked<ITEMS>(keys, wave_base, n, full, nvalid, ldk, kx);
+    load_blocked<ITEMS>(vals, wave_base, n, full, nvalid, ldv, x);
+    K kprev = shfl_up(kx[ITEMS - 1], 1);
+    // (a wave that starts at or past n has no items and reads nothing)
+    if (lane == 0 && wave_base > 0 && wave_base < n) kprev = (K)keys[wave_base - 1];
     unsigned heads = 0;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         if (j < nvalid) {
-            int e = (int)lane * ITEMS + j;
-            x[j] = ldv[WaveTile<ITEMS>::pad(e)];
-            K kc = ldk[WaveTile<ITEMS>::pad(e)];
-            bool head;
-            if (e > 0) head = !(bool)pred(ldk[WaveTile<ITEMS>::pad(e - 1)], kc);
-            else if (wave_base > 0) head = !(bool)pred(kprev_wave, kc);
-            else head = true;
+            bool head = (wave_base + (size_t)lane * ITEMS + j == 0) || !(bool)pred(j ? kx[j - 1] : kprev, kx[j]);
             heads |= (unsigned)head << j;
         }
     }

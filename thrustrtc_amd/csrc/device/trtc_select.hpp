@@ -28,39 +28,6 @@ namespace trtc {
 
 typedef unsigned long long SelC;
 
-// Valid items of this lane in a blocked layout (wave_base + lane*ITEMS + j).
-template <int ITEMS> TRTC_DEV int lane_valid(size_t wave_base, size_t n, bool full)
-{
-    if (full) return ITEMS;
-    size_t first = wave_base + (size_t)lane_id() * ITEMS;
-    return first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
-}
-
-// Blocked load of a wave's 64*ITEMS items: 16-byte vector loads of each
-// lane's run for contiguous views of 4/8-byte elements, else coalesced
-// striped loads transposed through the wave's LDS buffer `lds`.
-template <int ITEMS, class T, class View>
-TRTC_DEV void load_blocked(View& v, size_t wave_base, size_t n, bool full, int nvalid, T* lds, T (&x)[ITEMS])
-{
-    const unsigned lane = lane_id();
-    if constexpr (DirectVec<View, T, ITEMS>::value) {
-        if (full && (((size_t)Contig<View>::ptr(v)) & 15) == 0) {
-            load_run_vec<ITEMS>(Contig<View>::ptr(v) + wave_base + (size_t)lane * ITEMS, x);
-            return;
-        }
-    }
-    wave_sync();  // the buffer's previous contents are consumed
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        int i = k * 64 + (int)lane;
-        if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)v[wave_base + i];
-    }
-    wave_sync();
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++)
-        if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
-}
-
 // Offsets of a selection in which this thread selected `cnt` items: `local` =
 // its first slot in the tile, `tot` = the tile's count, `P` = the count of all
 // earlier tiles (lookback).  Publishes the tile's count; the last tile stores
@@ -148,7 +115,8 @@ TRTC_DEV void unique_tile(KIn& kin, VIn& vin, KOut& kout, VOut& vout, Pred& pred
     // key before this lane's first item: the previous lane's last (full
     // whenever this lane has an item), or the item before the wave
     K prev = shfl_up(x[ITEMS - 1], 1);
-    if (lane == 0 && wave_base > 0) prev = (K)kin[wave_base - 1];
+    // (a wave that starts at or past n has no items and reads nothing)
+    if (lane == 0 && wave_base > 0 && wave_base < n) prev = (K)kin[wave_base - 1];
     V v[VALS ? ITEMS : 1];
     if constexpr (VALS) load_blocked<ITEMS>(vin, wave_base, n, full, nvalid, (V*)s_raw[wave], v);
 
@@ -281,7 +249,7 @@ TRTC_DEV void reduce_by_key_tile(Keys& keys, Vals& vals, KOut& kout, VOut& vout,
     // neighbour keys: before this lane's first item and after its last
     K kprev = shfl_up(kx[ITEMS - 1], 1);
     K knext = shfl_down(kx[0], 1);
-    if (lane == 0 && wave_base > 0) kprev = (K)keys[wave_base - 1];
+    if (lane == 0 && wave_base > 0 && wave_base < n) kprev = (K)keys[wave_base - 1];
     if (lane == 63 && wave_base + WELEMS < n) knext = (K)keys[wave_base + WELEMS];
     load_blocked<ITEMS>(vals, wave_base, n, full, nvalid, (V*)s_raw[wave], x);
 

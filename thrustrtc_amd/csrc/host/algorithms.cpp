@@ -25,7 +25,7 @@ using namespace trtc_host;
 namespace {
 
 constexpr unsigned kBlock = 256;
-constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: trtc_scan_k and trtc_copy_if_k
+constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: scans, selections, by-key kernels
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -593,7 +593,7 @@ uint32_t general_reduce_by_key(const DVVectorLike& kin, const DVVectorLike& vin,
     args.view(pred);
     args.view(op);
     // state {V v; unsigned flag; unsigned tails;} fits align_up(V, 8) + 8 bytes
-    return run_selection(name, args, n, (size_t)kBlock * items_for(std::max(kin.elem_size(), vsz)), kBlock,
+    return run_selection(name, args, n, (size_t)kScanBlock * items_for(std::max(kin.elem_size(), vsz)), kScanBlock,
                          align_up(vsz, 8) + 8, false);
 }
 
@@ -618,7 +618,7 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kBlock * items_for(vsz);
+    size_t tile = (size_t)kScanBlock * items_for(std::max(keys.elem_size(), vsz));
     size_t tiles = (n + tile - 1) / tile;
     size_t seg_bytes = align_up(vsz + 4, std::min<size_t>(std::max<size_t>(vsz, 4), 8));
     LookbackScratch lb;
@@ -632,7 +632,7 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
     args.view(op);
     args.raw(initv.data(), initv.size());
     args.val(a);
-    return launch_kernel(k, dim3((unsigned)tiles), dim3(kBlock), args.data(), 0, s);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s);
 }
 
 }  // namespace
