@@ -3,10 +3,9 @@
 //
 // 1. Tile sort: a tile of BLOCK*ITEMS keys (with their input indices) is
 //    sorted in LDS: every thread sorts its ITEMS consecutive keys (odd-even
-//    transposition, stable), then runs are merged pairwise up to the tile size; a
-//    key's new position is its offset in its own run plus its rank in the
-//    partner run (lower bound for the left run, upper bound for the right
-//    run, so equal keys keep their order).
+//    transposition, stable), then runs are merged pairwise up to the tile
+//    size by merge path (each thread: one diagonal binary search, then ITEMS
+//    sequential merge steps; equal keys take the left run first).
 // 2. Merge passes: runs of width w are merged pairwise into runs of 2w.  Each
 //    workgroup produces one output tile: it finds where the tile's first and
 //    last output diagonals cut the two input runs (merge-path binary search,
@@ -17,27 +16,16 @@
 
 namespace trtc {
 
-// Number of a[0..len) that go before x (strictly less: comp(a, x)).
+// Merge-path split of diagonal d of runs A (lenA) and B (lenB): the number of
+// the first d outputs taken from A (equal keys: A first).
 template <class T, class Comp>
-TRTC_DEV unsigned lower_rank(const T* a, unsigned len, const T& x, Comp& comp)
+TRTC_DEV size_t merge_split(const T* A, size_t lenA, const T* B, size_t lenB, size_t d, Comp& comp)
 {
-    unsigned lo = 0, hi = len;
+    size_t lo = d > lenB ? d - lenB : 0;
+    size_t hi = d < lenA ? d : lenA;
     while (lo < hi) {
-        unsigned mid = (lo + hi) >> 1;
-        if (comp(a[mid], x)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Number of a[0..len) that go before or with x (not greater: !comp(x, a)).
-template <class T, class Comp>
-TRTC_DEV unsigned upper_rank(const T* a, unsigned len, const T& x, Comp& comp)
-{
-    unsigned lo = 0, hi = len;
-    while (lo < hi) {
-        unsigned mid = (lo + hi) >> 1;
-        if (!comp(x, a[mid])) lo = mid + 1;
+        size_t mid = (lo + hi) >> 1;
+        if (!comp(B[d - mid - 1], A[mid])) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -92,31 +80,46 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
             s_i[p0 + e] = ix[e];
         }
     __syncthreads();
-    // 2. pairwise run merges inside the tile
+    // 2. pairwise run merges inside the tile, by merge path: the thread that
+    //    owns outputs [p0, p0 + ITEMS) of a run pair finds where its first
+    //    diagonal cuts the two runs (one binary search, ties to the left run)
+    //    and merges its ITEMS outputs sequentially
     for (unsigned run = ITEMS; run < (unsigned)TILE; run <<= 1) {
-        unsigned pos[ITEMS];
+        unsigned cnt = 0;
+        if (p0 < valid) {
+            const unsigned g = p0 / (2 * run) * (2 * run);  // start of the run pair
+            const unsigned la = (g + run <= valid ? run : valid - g);
+            const unsigned lb = g + run < valid ? ((g + 2 * run <= valid ? 2 * run : valid - g) - run) : 0;
+            const unsigned d = p0 - g;
+            cnt = la + lb - d < (unsigned)ITEMS ? la + lb - d : (unsigned)ITEMS;
+            const T* A = s_k + g;
+            const T* B = s_k + g + la;
+            unsigned i = (unsigned)merge_split(A, la, B, lb, d, comp);
+            unsigned j = d - i;
 #pragma unroll
-        for (int e = 0; e < ITEMS; e++) {
-            const unsigned p = p0 + e;
-            pos[e] = p;
-            if ((unsigned)e >= mine) continue;
-            k[e] = s_k[p];
-            ix[e] = s_i[p];
-            const unsigned g = p / (2 * run) * (2 * run);  // start of the run pair
-            const unsigned o = p - g;
-            if (g + run >= valid) continue;  // no right run: position unchanged
-            const unsigned rlen = (g + 2 * run <= valid ? 2 * run : valid - g) - run;
-            if (o < run)
-                pos[e] = g + o + lower_rank(s_k + g + run, rlen, k[e], comp);
-            else
-                pos[e] = g + (o - run) + upper_rank(s_k + g, run, k[e], comp);
+            for (int e = 0; e < ITEMS; e++) {
+                if ((unsigned)e >= cnt) break;
+                bool takeA;
+                if (i >= la) takeA = false;
+                else if (j >= lb) takeA = true;
+                else takeA = !comp(B[j], A[i]);
+                if (takeA) {
+                    k[e] = A[i];
+                    ix[e] = s_i[g + i];
+                    i++;
+                } else {
+                    k[e] = B[j];
+                    ix[e] = s_i[g + la + j];
+                    j++;
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < ITEMS; e++)
-            if ((unsigned)e < mine) {
-                s_k[pos[e]] = k[e];
-                s_i[pos[e]] = ix[e];
+            if ((unsigned)e < cnt) {
+                s_k[p0 + e] = k[e];
+                s_i[p0 + e] = ix[e];
             }
         __syncthreads();
     }
@@ -124,21 +127,6 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
         keys[base + p] = s_k[p];
         if (a.with_idx) idx[base + p] = s_i[p];
     }
-}
-
-// Merge-path split of diagonal d of runs A (lenA) and B (lenB): the number of
-// the first d outputs taken from A (equal keys: A first).
-template <class T, class Comp>
-TRTC_DEV size_t merge_split(const T* A, size_t lenA, const T* B, size_t lenB, size_t d, Comp& comp)
-{
-    size_t lo = d > lenB ? d - lenB : 0;
-    size_t hi = d < lenA ? d : lenA;
-    while (lo < hi) {
-        size_t mid = (lo + hi) >> 1;
-        if (!comp(B[d - mid - 1], A[mid])) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
 }
 
 template <int BLOCK, int ITEMS, class T, class Comp>
