@@ -135,6 +135,6 @@ def test_merge_sort_kernels_compile_for_user_comparator(nm):
     n = nm.native
     f = n.n_functor_create(_ptrs(nm, []), _strs(nm, []), _strs(nm, ["x", "y"]), b"        return (x & 255) < (y & 255);\n")
     name = n.n_dv_name_view_cls(f).decode()
-    for k in ("trtc_msort_tile_k", "trtc_msort_merge_k"):
+    for k in ("trtc_msort_tile_k", "trtc_msort_merge_k", "trtc_msort_split_k"):
         assert n.n_jit_compile_check(b"gfx950", f"{k}<uint32_t, {name}>".encode()) == 0
     assert n.n_jit_compile_check(b"gfx950", b"trtc_msort_merge_k<double, Greater>") == 0

@@ -79,8 +79,9 @@ struct RadixPassArgs
 struct MergeSortArgs
 {
     size_t n;
-    size_t width;  // run length of the input of a merge pass
-    int with_idx;  // carry the input index of every key
+    size_t width;                // run length of the input of a merge pass
+    int with_idx;                // carry the input index of every key
+    unsigned long long* splits;  // per merge tile: its first diagonal's split (null: search in the merge)
 };
 
 // ---- batched segments: many independent sequences in one launch --------

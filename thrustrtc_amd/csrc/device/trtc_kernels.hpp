@@ -182,3 +182,9 @@ trtc_msort_merge_k(const T* kin, const unsigned* iin, T* kout, unsigned* iout, C
 {
     trtc::msort_merge<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, T>(kin, iin, kout, iout, comp, a);
 }
+
+template <class T, class Comp>
+__global__ void __launch_bounds__(256) trtc_msort_split_k(const T* kin, Comp comp, trtc::MergeSortArgs a)
+{
+    trtc::msort_splits<TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS, T>(kin, comp, a);
+}

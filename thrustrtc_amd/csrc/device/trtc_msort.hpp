@@ -129,6 +129,28 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
     }
 }
 
+// The run pair of output position out0 in a merge pass over runs of width w:
+// pair start, lengths of its left (A) and right (B) runs.
+TRTC_DEV void msort_pair(size_t out0, size_t n, size_t w, size_t& pair, size_t& lenA, size_t& lenB)
+{
+    pair = out0 / (2 * w) * (2 * w);
+    lenA = pair + w < n ? w : n - pair;
+    lenB = pair + w < n ? ((pair + 2 * w <= n ? 2 * w : n - pair) - w) : 0;
+}
+
+// Splits of every merge tile's first diagonal, one thread per tile: all the
+// binary searches of a pass run at once instead of heading each workgroup.
+template <int TILE, class T, class Comp>
+TRTC_DEV void msort_splits(const T* kin, Comp& comp, const MergeSortArgs& a)
+{
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tiles = (a.n + TILE - 1) / TILE;
+    if (b >= tiles) return;
+    size_t pair, lenA, lenB;
+    msort_pair(b * TILE, a.n, a.width, pair, lenA, lenB);
+    a.splits[b] = merge_split(kin + pair, lenA, kin + pair + lenA, lenB, b * TILE - pair, comp);
+}
+
 template <int BLOCK, int ITEMS, class T, class Comp>
 TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* iout, Comp& comp,
                           const MergeSortArgs& a)
@@ -137,16 +159,23 @@ TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* 
     __shared__ T s_k[TILE];
     __shared__ unsigned s_i[TILE];
     __shared__ size_t s_split[2];
-    const size_t n = a.n, w = a.width;
     const size_t out0 = (size_t)blockIdx.x * TILE;
-    const size_t pair = out0 / (2 * w) * (2 * w);
-    const size_t lenA = (pair + w < n ? w : n - pair);
-    const size_t lenB = pair + w < n ? ((pair + 2 * w <= n ? 2 * w : n - pair) - w) : 0;
+    size_t pair, lenA, lenB;
+    msort_pair(out0, a.n, a.width, pair, lenA, lenB);
     const T* A = kin + pair;
     const T* B = kin + pair + lenA;
     const size_t d0 = out0 - pair;
     const size_t d1 = (d0 + TILE < lenA + lenB) ? d0 + TILE : lenA + lenB;
-    if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split(A, lenA, B, lenB, threadIdx.x ? d1 : d0, comp);
+    if (threadIdx.x < 2) {
+        size_t sp;
+        if (!a.splits)
+            sp = merge_split(A, lenA, B, lenB, threadIdx.x ? d1 : d0, comp);
+        else if (threadIdx.x == 0)
+            sp = a.splits[blockIdx.x];
+        else  // the next tile starts at d1 in the same pair, or the pair ends at d1
+            sp = d1 == lenA + lenB ? lenA : a.splits[blockIdx.x + 1];
+        s_split[threadIdx.x] = sp;
+    }
     __syncthreads();
     const size_t a0 = s_split[0], a1 = s_split[1];
     const size_t b0 = d0 - a0, b1 = d1 - a1;

@@ -210,14 +210,17 @@ bool merge_sort(void* keys, const std::string& T, size_t esize, size_t n, const 
     const std::string cc = vcls(comp);
     const std::string tname = "trtc_msort_tile_k<" + T + ", " + cc + ">";
     const std::string mname = "trtc_msort_merge_k<" + T + ", " + cc + ">";
-    KernelRef tk, mk;
+    const std::string sname = "trtc_msort_split_k<" + T + ", " + cc + ">";
+    KernelRef tk, mk, sk;
     if (!get_template_kernel(tname, tname, tk)) return false;
     const bool with_idx = vals != nullptr;
     PoolBuf ia(with_idx ? n * 4 : 0, s), ib(with_idx ? n * 4 : 0, s), kb(n * esize, s);
     if ((with_idx && (!ia.p || !ib.p)) || !kb.p) return false;
     if (with_idx && !iota_u32(ia.as<unsigned>(), n, s)) return false;
-    trtc::MergeSortArgs a{n, kMsortTile, with_idx ? 1 : 0};
     const unsigned tiles = (unsigned)((n + kMsortTile - 1) / kMsortTile);
+    PoolBuf splits(n > kMsortTile ? (size_t)(tiles + 1) * 8 : 0, s);
+    if (n > kMsortTile && !splits.p) return false;
+    trtc::MergeSortArgs a{n, kMsortTile, with_idx ? 1 : 0, nullptr};
     {
         Args args;
         args.val(keys);
@@ -230,9 +233,18 @@ bool merge_sort(void* keys, const std::string& T, size_t esize, size_t n, const 
     void* ibuf[2] = {ia.p, ib.p};
     int cur = 0;
     if (n > kMsortTile) {
-        if (!get_template_kernel(mname, mname, mk)) return false;
+        if (!get_template_kernel(mname, mname, mk) || !get_template_kernel(sname, sname, sk)) return false;
+        a.splits = splits.as<unsigned long long>();
         for (size_t w = kMsortTile; w < n; w *= 2) {
             a.width = w;
+            {
+                // every tile's split first (all searches of the pass at once)
+                Args sargs;
+                sargs.val(kbuf[cur]);
+                sargs.view(comp);
+                sargs.val(a);
+                if (!launch_kernel(sk, dim3((tiles + 255) / 256), dim3(256), sargs.data(), 0, s)) return false;
+            }
             Args args;
             args.val(kbuf[cur]);
             args.val(ibuf[cur]);
