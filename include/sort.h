@@ -25,4 +25,9 @@ bool THRUST_RTC_API TRTC_Segmented_Sort_By_Key(DVVectorLike& keys, DVVectorLike&
                                                const size_t* offsets, size_t num_segments,
                                                const Functor& comp);
 
+
+// Whether the vector is sorted / the length of its sorted prefix (sort.h:16-19).
+bool THRUST_RTC_API TRTC_Is_Sorted_Until(const DVVectorLike& vec, size_t& result);
+bool THRUST_RTC_API TRTC_Is_Sorted_Until(const DVVectorLike& vec, const Functor& comp, size_t& result);
+
 #endif

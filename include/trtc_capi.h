@@ -146,6 +146,50 @@ TRTC_CAPI unsigned n_partition_stencil(void* ptr_vec, void* ptr_stencil, void* p
 TRTC_CAPI unsigned n_partition_copy(void* ptr_vec_in, void* ptr_vec_true, void* ptr_vec_false, void* ptr_pred); /* api.h:167 */
 TRTC_CAPI unsigned n_partition_copy_stencil(void* ptr_vec_in, void* ptr_stencil, void* ptr_vec_true, void* ptr_vec_false, void* ptr_pred); /* api.h:168 */
 
+/* ---- the other algorithms (api.h:87-186) ---- */
+TRTC_CAPI void* n_dvtuple_create(void* ptr_dvs, void* ptr_names); /* api.h:87 */
+TRTC_CAPI int n_replace(void* ptr_vec, void* ptr_old_value, void* ptr_new_value); /* api.h:105 */
+TRTC_CAPI int n_replace_if(void* ptr_vec, void* p_pred, void* ptr_new_value); /* api.h:106 */
+TRTC_CAPI int n_replace_copy(void* ptr_in, void* ptr_out, void* ptr_old_value, void* ptr_new_value); /* api.h:107 */
+TRTC_CAPI int n_replace_copy_if(void* ptr_in, void* ptr_out, void* p_pred, void* ptr_new_value); /* api.h:108 */
+TRTC_CAPI int n_for_each(void* ptr_vec, void* ptr_f); /* api.h:109 */
+TRTC_CAPI int n_adjacent_difference(void* ptr_in, void* ptr_out, void* ptr_binary_op); /* api.h:110 */
+TRTC_CAPI int n_sequence(void* ptr_vec, void* ptr_value_init, void* ptr_value_step); /* api.h:111 */
+TRTC_CAPI int n_tabulate(void* ptr_vec, void* ptr_op); /* api.h:112 */
+TRTC_CAPI int n_transform(void* ptr_in, void* ptr_out, void* ptr_op); /* api.h:113 */
+TRTC_CAPI int n_transform_binary(void* ptr_in1, void* ptr_in2, void* ptr_out, void* ptr_op); /* api.h:114 */
+TRTC_CAPI int n_transform_if(void* ptr_in, void* ptr_out, void* ptr_op, void* ptr_pred); /* api.h:115 */
+TRTC_CAPI int n_transform_if_stencil(void* ptr_in, void* ptr_stencil, void* ptr_out, void* ptr_op, void* ptr_pred); /* api.h:116 */
+TRTC_CAPI int n_transform_binary_if_stencil(void* ptr_in1, void* ptr_in2, void* ptr_stencil, void* ptr_out, void* ptr_op, void* ptr_pred); /* api.h:117 */
+TRTC_CAPI int n_gather(void* ptr_map, void* ptr_in, void* ptr_out); /* api.h:120 */
+TRTC_CAPI int n_gather_if(void* ptr_map, void* ptr_stencil, void* ptr_in, void* ptr_out, void* ptr_pred); /* api.h:121 */
+TRTC_CAPI int n_scatter(void* ptr_in, void* ptr_map, void* ptr_out); /* api.h:122 */
+TRTC_CAPI int n_scatter_if(void* ptr_in, void* ptr_map, void* ptr_stencil, void* ptr_out, void* ptr_pred); /* api.h:123 */
+TRTC_CAPI int n_swap(void* ptr_vec1, void* ptr_vec2); /* api.h:125 */
+TRTC_CAPI int n_equal(void* ptr_vec1, void* ptr_vec2, void* ptr_binary_pred); /* api.h:132; 1 / 0, -1 on error */
+TRTC_CAPI unsigned long long n_min_element(void* ptr_vec, void* ptr_comp); /* api.h:133 */
+TRTC_CAPI unsigned long long n_max_element(void* ptr_vec, void* ptr_comp); /* api.h:134 */
+TRTC_CAPI int n_minmax_element(void* ptr_vec, void* ptr_comp, unsigned long long* ret); /* api.h:135 */
+TRTC_CAPI void* n_inner_product(void* ptr_vec1, void* ptr_vec2, void* ptr_init, void* ptr_binary_op1, void* ptr_binary_op2); /* api.h:136; new DV */
+TRTC_CAPI int n_all_of(void* ptr_vec, void* ptr_pred); /* api.h:138 */
+TRTC_CAPI int n_any_of(void* ptr_vec, void* ptr_pred); /* api.h:139 */
+TRTC_CAPI int n_none_of(void* ptr_vec, void* ptr_pred); /* api.h:140 */
+TRTC_CAPI int n_is_partitioned(void* ptr_vec, void* ptr_pred); /* api.h:141 */
+TRTC_CAPI long long n_find(void* ptr_vec, void* ptr_value); /* api.h:171; vec size when absent, -1 on error */
+TRTC_CAPI long long n_find_if(void* ptr_vec, void* ptr_pred); /* api.h:172 */
+TRTC_CAPI long long n_find_if_not(void* ptr_vec, void* ptr_pred); /* api.h:173 */
+TRTC_CAPI long long n_mismatch(void* ptr_vec1, void* ptr_vec2, void* ptr_pred); /* api.h:174 */
+TRTC_CAPI unsigned long long n_lower_bound(void* ptr_vec, void* ptr_value, void* ptr_comp); /* api.h:175 */
+TRTC_CAPI unsigned long long n_upper_bound(void* ptr_vec, void* ptr_value, void* ptr_comp); /* api.h:176 */
+TRTC_CAPI int n_binary_search(void* ptr_vec, void* ptr_value, void* ptr_comp); /* api.h:177 */
+TRTC_CAPI int n_lower_bound_v(void* ptr_vec, void* ptr_values, void* ptr_result, void* ptr_comp); /* api.h:178 */
+TRTC_CAPI int n_upper_bound_v(void* ptr_vec, void* ptr_values, void* ptr_result, void* ptr_comp); /* api.h:179 */
+TRTC_CAPI int n_binary_search_v(void* ptr_vec, void* ptr_values, void* ptr_result, void* ptr_comp); /* api.h:180 */
+TRTC_CAPI unsigned long long n_partition_point(void* ptr_vec, void* ptr_pred); /* api.h:181 */
+TRTC_CAPI unsigned long long n_is_sorted_until(void* ptr_vec, void* ptr_comp); /* api.h:182 */
+TRTC_CAPI int n_merge(void* ptr_vec1, void* ptr_vec2, void* ptr_vec_out, void* ptr_comp); /* api.h:185 */
+TRTC_CAPI int n_merge_by_key(void* ptr_keys1, void* ptr_keys2, void* ptr_value1, void* ptr_value2, void* ptr_keys_out, void* ptr_value_out, void* ptr_comp); /* api.h:186 */
+
 /* ---- extensions: no counterpart in the reference ----
  * Batched segmented primitives (one launch per kernel stage for many
  * independent segments) and the graph-captured batch.  Callers: workloads

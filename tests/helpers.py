@@ -35,6 +35,11 @@ def native_module():
     return _load_path("trtc_native_standalone", os.path.join(PKG, "native.py"))
 
 
+def oracle_algorithms():
+    """oracle/algorithms.py (the CPU restatement of the other algorithms)."""
+    return _load_path("trtc_oracle_algorithms", os.path.join(ROOT, "oracle", "algorithms.py"))
+
+
 def workloads():
     ensure_built()
     return _load_path("trtc_workloads_standalone", os.path.join(PKG, "workloads.py"))

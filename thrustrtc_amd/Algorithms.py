@@ -160,6 +160,190 @@ def Fill(vec, value):
     check_i(native.n_fill(vec.m_cptr, value.m_cptr))
 
 
+
+# ---- Transformations.py / Copying.py (the other element-wise algorithms)
+def Replace(vec, old_value, new_value):
+    check_i(native.n_replace(vec.m_cptr, old_value.m_cptr, new_value.m_cptr))
+
+
+def Replace_If(vec, pred, new_value):
+    check_i(native.n_replace_if(vec.m_cptr, pred.m_cptr, new_value.m_cptr))
+
+
+def Replace_Copy(vec_in, vec_out, old_value, new_value):
+    check_i(native.n_replace_copy(vec_in.m_cptr, vec_out.m_cptr, old_value.m_cptr, new_value.m_cptr))
+
+
+def Replace_Copy_If(vec_in, vec_out, pred, new_value):
+    check_i(native.n_replace_copy_if(vec_in.m_cptr, vec_out.m_cptr, pred.m_cptr, new_value.m_cptr))
+
+
+def For_Each(vec, f):
+    check_i(native.n_for_each(vec.m_cptr, f.m_cptr))
+
+
+def Adjacent_Difference(vec_in, vec_out, binary_op=None):
+    check_i(native.n_adjacent_difference(vec_in.m_cptr, vec_out.m_cptr, _p(binary_op)))
+
+
+def Sequence(vec, value_init=None, value_step=None):
+    check_i(native.n_sequence(vec.m_cptr, _p(value_init), _p(value_step)))
+
+
+def Tabulate(vec, op):
+    check_i(native.n_tabulate(vec.m_cptr, op.m_cptr))
+
+
+def Transform(vec_in, vec_out, op):
+    check_i(native.n_transform(vec_in.m_cptr, vec_out.m_cptr, op.m_cptr))
+
+
+def Transform_Binary(vec_in1, vec_in2, vec_out, op):
+    check_i(native.n_transform_binary(vec_in1.m_cptr, vec_in2.m_cptr, vec_out.m_cptr, op.m_cptr))
+
+
+def Transform_If(vec_in, vec_out, op, pred):
+    check_i(native.n_transform_if(vec_in.m_cptr, vec_out.m_cptr, op.m_cptr, pred.m_cptr))
+
+
+def Transform_If_Stencil(vec_in, vec_stencil, vec_out, op, pred):
+    check_i(native.n_transform_if_stencil(vec_in.m_cptr, vec_stencil.m_cptr, vec_out.m_cptr, op.m_cptr, pred.m_cptr))
+
+
+def Transform_Binary_If_Stencil(vec_in1, vec_in2, vec_stencil, vec_out, op, pred):
+    check_i(native.n_transform_binary_if_stencil(vec_in1.m_cptr, vec_in2.m_cptr, vec_stencil.m_cptr, vec_out.m_cptr,
+                                                 op.m_cptr, pred.m_cptr))
+
+
+def Gather(vec_map, vec_in, vec_out):
+    check_i(native.n_gather(vec_map.m_cptr, vec_in.m_cptr, vec_out.m_cptr))
+
+
+def Gather_If(vec_map, vec_stencil, vec_in, vec_out, pred=None):
+    check_i(native.n_gather_if(vec_map.m_cptr, vec_stencil.m_cptr, vec_in.m_cptr, vec_out.m_cptr, _p(pred)))
+
+
+def Scatter(vec_in, vec_map, vec_out):
+    check_i(native.n_scatter(vec_in.m_cptr, vec_map.m_cptr, vec_out.m_cptr))
+
+
+def Scatter_If(vec_in, vec_map, vec_stencil, vec_out, pred=None):
+    check_i(native.n_scatter_if(vec_in.m_cptr, vec_map.m_cptr, vec_stencil.m_cptr, vec_out.m_cptr, _p(pred)))
+
+
+def Swap(vec1, vec2):
+    check_i(native.n_swap(vec1.m_cptr, vec2.m_cptr))
+
+
+# ---- Reductions.py (the other reductions)
+def _bool(res):
+    return None if res < 0 else res != 0
+
+
+def Equal(vec1, vec2, binary_pred=None):
+    return _bool(check_i(native.n_equal(vec1.m_cptr, vec2.m_cptr, _p(binary_pred))))
+
+
+def Min_Element(vec, comp=None):
+    return check_ull(native.n_min_element(vec.m_cptr, _p(comp)))
+
+
+def Max_Element(vec, comp=None):
+    return check_ull(native.n_max_element(vec.m_cptr, _p(comp)))
+
+
+def MinMax_Element(vec, comp=None):
+    arr = (ctypes.c_ulonglong * 2)()
+    check_i(native.n_minmax_element(vec.m_cptr, _p(comp), arr))
+    return (arr[0], arr[1])
+
+
+def Inner_Product(vec1, vec2, value_init, binary_op1=None, binary_op2=None):
+    dvptr = check_cptr(native.n_inner_product(vec1.m_cptr, vec2.m_cptr, value_init.m_cptr, _p(binary_op1),
+                                              _p(binary_op2)))
+    ret = ValueFromDVPtr(dvptr)
+    native.n_dv_destroy(dvptr)
+    return ret
+
+
+def All_Of(vec, pred):
+    return _bool(check_i(native.n_all_of(vec.m_cptr, pred.m_cptr)))
+
+
+def Any_Of(vec, pred):
+    return _bool(check_i(native.n_any_of(vec.m_cptr, pred.m_cptr)))
+
+
+def None_Of(vec, pred):
+    return _bool(check_i(native.n_none_of(vec.m_cptr, pred.m_cptr)))
+
+
+def Is_Partitioned(vec, pred):
+    return _bool(check_i(native.n_is_partitioned(vec.m_cptr, pred.m_cptr)))
+
+
+# ---- Searching.py
+def _index(res, vec):
+    return None if res >= vec.size() else res
+
+
+def Find(vec, value):
+    return _index(check_i(native.n_find(vec.m_cptr, value.m_cptr)), vec)
+
+
+def Find_If(vec, pred):
+    return _index(check_i(native.n_find_if(vec.m_cptr, pred.m_cptr)), vec)
+
+
+def Find_If_Not(vec, pred):
+    return _index(check_i(native.n_find_if_not(vec.m_cptr, pred.m_cptr)), vec)
+
+
+def Mismatch(vec1, vec2, pred=None):
+    return _index(check_i(native.n_mismatch(vec1.m_cptr, vec2.m_cptr, _p(pred))), vec1)
+
+
+def Lower_Bound(vec, value, comp=None):
+    return check_ull(native.n_lower_bound(vec.m_cptr, value.m_cptr, _p(comp)))
+
+
+def Upper_Bound(vec, value, comp=None):
+    return check_ull(native.n_upper_bound(vec.m_cptr, value.m_cptr, _p(comp)))
+
+
+def Binary_Search(vec, value, comp=None):
+    return _bool(check_i(native.n_binary_search(vec.m_cptr, value.m_cptr, _p(comp))))
+
+
+def Partition_Point(vec, pred):
+    return check_ull(native.n_partition_point(vec.m_cptr, pred.m_cptr))
+
+
+def Is_Sorted_Until(vec, comp=None):
+    return check_ull(native.n_is_sorted_until(vec.m_cptr, _p(comp)))
+
+
+def Lower_Bound_V(vec, values, result, comp=None):
+    check_i(native.n_lower_bound_v(vec.m_cptr, values.m_cptr, result.m_cptr, _p(comp)))
+
+
+def Upper_Bound_V(vec, values, result, comp=None):
+    check_i(native.n_upper_bound_v(vec.m_cptr, values.m_cptr, result.m_cptr, _p(comp)))
+
+
+def Binary_Search_V(vec, values, result, comp=None):
+    check_i(native.n_binary_search_v(vec.m_cptr, values.m_cptr, result.m_cptr, _p(comp)))
+
+
+# ---- Merging.py
+def Merge(vec1, vec2, vec_out, comp=None):
+    check_i(native.n_merge(vec1.m_cptr, vec2.m_cptr, vec_out.m_cptr, _p(comp)))
+
+
+def Merge_By_Key(keys1, keys2, value1, value2, keys_out, value_out, comp=None):
+    check_i(native.n_merge_by_key(keys1.m_cptr, keys2.m_cptr, value1.m_cptr, value2.m_cptr, keys_out.m_cptr,
+                                  value_out.m_cptr, _p(comp)))
+
 # ---- extensions: batched segmented primitives
 def _offsets(offsets):
     arr = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))

@@ -4,6 +4,7 @@ objects it views alive for as long as it lives."""
 import ctypes
 from .native import native, check_cptr
 from .DVVector import DVVectorLike
+from .DeviceViewable import DeviceViewable
 from .utils import ObjArray, StrArray
 
 
@@ -78,3 +79,15 @@ class DVCustomVector(DVVectorLike):
         self.m_cptr = check_cptr(native.n_dvcustomvector_create(
             o_elems.m_cptr, o_names.m_cptr, name_idx.encode("utf-8"), code_body.encode("utf-8"),
             elem_cls.encode("utf-8"), ctypes.c_ulonglong(size).value, 1 if read_only else 0))
+
+
+class DVTuple(DeviceViewable):
+    """A struct of named device-viewable values (reference:
+    python/ThrustRTC/DVTuple.py); the same members as a DVZipped element make
+    the same struct, so it can be copied into zipped vectors."""
+
+    def __init__(self, elem_map):
+        self.m_elem_map = elem_map
+        o_names = StrArray(list(elem_map.keys()))
+        o_elems = ObjArray(list(elem_map.values()))
+        self.m_cptr = check_cptr(native.n_dvtuple_create(o_elems.m_cptr, o_names.m_cptr))

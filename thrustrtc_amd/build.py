@@ -38,10 +38,11 @@ DEVICE_HEADERS = [
     "trtc_radix.hpp",
     "trtc_msort.hpp",
     "trtc_select.hpp",
+    "trtc_merge.hpp",
     "trtc_kernels.hpp",
 ]
 
-HOST_SOURCES = ["context.cpp", "dvvector.cpp", "fake_vectors.cpp", "algorithms.cpp", "batch.cpp", "segmented.cpp", "server.cpp", "capi.cpp"]
+HOST_SOURCES = ["context.cpp", "dvvector.cpp", "fake_vectors.cpp", "algorithms.cpp", "algorithms_ext.cpp", "batch.cpp", "segmented.cpp", "server.cpp", "capi.cpp"]
 
 LIB = os.path.join(PKG, "libPyThrustRTC.so")
 WORKLOADS_LIB = os.path.join(PKG, "libtrtc_workloads.so")

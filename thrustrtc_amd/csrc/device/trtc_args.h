@@ -76,6 +76,12 @@ struct RadixPassArgs
     int descending;
 };
 
+struct MergeArgs
+{
+    size_t na;  // length of the first sorted input
+    size_t nb;  // length of the second
+};
+
 struct MergeSortArgs
 {
     size_t n;

@@ -218,3 +218,40 @@ struct TransformView
     TRTC_DEV size_t size() const { return _view_vec_in.size(); }
     TRTC_DEV ref_t operator[](size_t idx) const { return (value_t)_view_op(_view_vec_in[idx]); }
 };
+
+////////// Binary searches over sorted views (built_in.h binary-search routines) ///////////
+// The position of the first element of `vec` that is not before `value`
+// (comp(vec[i], value) false), std::lower_bound.
+template <class TVec, class T, class TComp>
+TRTC_DEV size_t d_lower_bound(TVec& vec, const T& value, TComp& comp)
+{
+    size_t lo = 0, hi = vec.size();
+    while (lo < hi) {
+        size_t mid = (lo + hi) >> 1;
+        if (comp(vec[mid], value)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The position of the first element that is after `value` (comp(value,
+// vec[i]) true), std::upper_bound.
+template <class TVec, class T, class TComp>
+TRTC_DEV size_t d_upper_bound(TVec& vec, const T& value, TComp& comp)
+{
+    size_t lo = 0, hi = vec.size();
+    while (lo < hi) {
+        size_t mid = (lo + hi) >> 1;
+        if (!comp(value, vec[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Whether an element equivalent to `value` is present, std::binary_search.
+template <class TVec, class T, class TComp>
+TRTC_DEV bool d_binary_search(TVec& vec, const T& value, TComp& comp)
+{
+    size_t i = d_lower_bound(vec, value, comp);
+    return i < vec.size() && !comp(value, vec[i]);
+}

@@ -14,6 +14,7 @@
 #include "trtc_radix.hpp"
 #include "trtc_msort.hpp"
 #include "trtc_select.hpp"
+#include "trtc_merge.hpp"
 
 #define TRTC_BLOCK 256
 
@@ -187,4 +188,20 @@ template <class T, class Comp>
 __global__ void __launch_bounds__(256) trtc_msort_split_k(const T* kin, Comp comp, trtc::MergeSortArgs a)
 {
     trtc::msort_splits<TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS, T>(kin, comp, a);
+}
+
+// Merge / Merge_By_Key of two sorted inputs (views of any kind).
+template <class AK, class BK, class OK, class Comp>
+__global__ void __launch_bounds__(TRTC_MSORT_BLOCK) trtc_merge_k(AK ak, BK bk, OK ok, Comp comp, trtc::MergeArgs a)
+{
+    typedef typename AK::value_t K;
+    trtc::merge_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, false, K>(ak, bk, ak, bk, ok, ok, comp, a);
+}
+
+template <class AK, class BK, class AV, class BV, class OK, class OV, class Comp>
+__global__ void __launch_bounds__(TRTC_MSORT_BLOCK)
+trtc_merge_by_key_k(AK ak, BK bk, AV av, BV bv, OK ok, OV ov, Comp comp, trtc::MergeArgs a)
+{
+    typedef typename AK::value_t K;
+    trtc::merge_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, true, K>(ak, bk, av, bv, ok, ov, comp, a);
 }

@@ -650,6 +650,11 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
 }  // namespace
 
 namespace trtc_host {
+bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Functor& op, ViewBuf& ret)
+{
+    return general_reduce(vec, init, op, ret);
+}
+
 bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,
                      int kind, bool descending, int val_bytes, hipStream_t s)
 {

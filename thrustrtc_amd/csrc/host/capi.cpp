@@ -5,6 +5,25 @@
 #include <cstring>
 
 #include "copy.h"
+#include "DVTuple.h"
+#include "adjacent_difference.h"
+#include "binary_search.h"
+#include "equal.h"
+#include "extrema.h"
+#include "fill.h"
+#include "find.h"
+#include "for_each.h"
+#include "gather.h"
+#include "inner_product.h"
+#include "logical.h"
+#include "merge.h"
+#include "mismatch.h"
+#include "replace.h"
+#include "scatter.h"
+#include "sequence.h"
+#include "swap.h"
+#include "tabulate.h"
+#include "transform.h"
 #include "count.h"
 #include "partition.h"
 #include "remove.h"
@@ -285,12 +304,7 @@ void* n_built_in_functor_create(const char* name) { return new Functor(name); }
 // ---- algorithms
 int n_fill(void* ptr_vec, void* ptr_value)
 {
-    // reference fill.cpp: vec[idx] = value for every element
-    static TRTC_For s_for({"view_vec", "value"}, "idx",
-                          "    view_vec[idx] = (decltype(view_vec)::value_t)value;\n");
-    DVVectorLike* vec = (DVVectorLike*)ptr_vec;
-    const DeviceViewable* args[] = {vec, (DeviceViewable*)ptr_value};
-    return s_for.launch_n(vec->size(), args) ? 0 : -1;
+    return TRTC_Fill(*(DVVectorLike*)ptr_vec, *(DeviceViewable*)ptr_value) ? 0 : -1;
 }
 
 int n_copy(void* ptr_in, void* ptr_out)
@@ -496,6 +510,196 @@ unsigned n_partition_copy_stencil(void* ptr_vec_in, void* ptr_stencil, void* ptr
     return TRTC_Partition_Copy_Stencil(*(DVL*)ptr_vec_in, *(DVL*)ptr_stencil, *(DVL*)ptr_vec_true,
                                        *(DVL*)ptr_vec_false, *(Functor*)ptr_pred);
 }
+
+// ---- the other algorithms (reference python/api_Transformations.cpp,
+// api_Copying.cpp, api_Reductions.cpp, api_Searching.cpp, api_Merging.cpp,
+// api_DVTuple.cpp)
+static inline int ok_i(bool b) { return b ? 0 : -1; }
+static inline int bool_i(bool ok, bool v) { return ok ? (v ? 1 : 0) : -1; }
+#define F(p) (*(Functor*)(p))
+#define V(p) (*(DVL*)(p))
+#define D(p) (*(DeviceViewable*)(p))
+
+void* n_dvtuple_create(void* ptr_dvs, void* ptr_names)
+{
+    const PtrArray& dvs = *(PtrArray*)ptr_dvs;
+    const StrArray& names = *(StrArray*)ptr_names;
+    if (dvs.size() != names.size()) return nullptr;
+    std::vector<CapturedDeviceViewable> m(dvs.size());
+    for (size_t i = 0; i < dvs.size(); i++) m[i] = {names[i].c_str(), dvs[i]};
+    return new DVTuple(m);
+}
+int n_replace(void* v, void* o, void* nv) { return ok_i(TRTC_Replace(V(v), D(o), D(nv))); }
+int n_replace_if(void* v, void* p, void* nv) { return ok_i(TRTC_Replace_If(V(v), F(p), D(nv))); }
+int n_replace_copy(void* i, void* o, void* ov, void* nv) { return ok_i(TRTC_Replace_Copy(V(i), V(o), D(ov), D(nv))); }
+int n_replace_copy_if(void* i, void* o, void* p, void* nv) { return ok_i(TRTC_Replace_Copy_If(V(i), V(o), F(p), D(nv))); }
+int n_for_each(void* v, void* f) { return ok_i(TRTC_For_Each(V(v), F(f))); }
+int n_adjacent_difference(void* i, void* o, void* op)
+{
+    return ok_i(op ? TRTC_Adjacent_Difference(V(i), V(o), F(op)) : TRTC_Adjacent_Difference(V(i), V(o)));
+}
+int n_sequence(void* v, void* init, void* step)
+{
+    if (!init) return ok_i(TRTC_Sequence(V(v)));
+    if (!step) return ok_i(TRTC_Sequence(V(v), D(init)));
+    return ok_i(TRTC_Sequence(V(v), D(init), D(step)));
+}
+int n_tabulate(void* v, void* op) { return ok_i(TRTC_Tabulate(V(v), F(op))); }
+int n_transform(void* i, void* o, void* op) { return ok_i(TRTC_Transform(V(i), V(o), F(op))); }
+int n_transform_binary(void* i1, void* i2, void* o, void* op) { return ok_i(TRTC_Transform_Binary(V(i1), V(i2), V(o), F(op))); }
+int n_transform_if(void* i, void* o, void* op, void* p) { return ok_i(TRTC_Transform_If(V(i), V(o), F(op), F(p))); }
+int n_transform_if_stencil(void* i, void* s, void* o, void* op, void* p)
+{
+    return ok_i(TRTC_Transform_If_Stencil(V(i), V(s), V(o), F(op), F(p)));
+}
+int n_transform_binary_if_stencil(void* i1, void* i2, void* s, void* o, void* op, void* p)
+{
+    return ok_i(TRTC_Transform_Binary_If_Stencil(V(i1), V(i2), V(s), V(o), F(op), F(p)));
+}
+int n_gather(void* m, void* i, void* o) { return ok_i(TRTC_Gather(V(m), V(i), V(o))); }
+int n_gather_if(void* m, void* s, void* i, void* o, void* p)
+{
+    return ok_i(p ? TRTC_Gather_If(V(m), V(s), V(i), V(o), F(p)) : TRTC_Gather_If(V(m), V(s), V(i), V(o)));
+}
+int n_scatter(void* i, void* m, void* o) { return ok_i(TRTC_Scatter(V(i), V(m), V(o))); }
+int n_scatter_if(void* i, void* m, void* s, void* o, void* p)
+{
+    return ok_i(p ? TRTC_Scatter_If(V(i), V(m), V(s), V(o), F(p)) : TRTC_Scatter_If(V(i), V(m), V(s), V(o)));
+}
+int n_swap(void* a, void* b) { return ok_i(TRTC_Swap(V(a), V(b))); }
+int n_equal(void* a, void* b, void* p)
+{
+    bool r = false;
+    bool ok = p ? TRTC_Equal(V(a), V(b), F(p), r) : TRTC_Equal(V(a), V(b), r);
+    return bool_i(ok, r);
+}
+unsigned long long n_min_element(void* v, void* c)
+{
+    size_t r;
+    return (c ? TRTC_Min_Element(V(v), F(c), r) : TRTC_Min_Element(V(v), r)) ? r : ~0ull;
+}
+unsigned long long n_max_element(void* v, void* c)
+{
+    size_t r;
+    return (c ? TRTC_Max_Element(V(v), F(c), r) : TRTC_Max_Element(V(v), r)) ? r : ~0ull;
+}
+int n_minmax_element(void* v, void* c, unsigned long long* ret)
+{
+    size_t a, b;
+    bool ok = c ? TRTC_MinMax_Element(V(v), F(c), a, b) : TRTC_MinMax_Element(V(v), a, b);
+    if (!ok) return -1;
+    ret[0] = a;
+    ret[1] = b;
+    return 0;
+}
+void* n_inner_product(void* a, void* b, void* init, void* op1, void* op2)
+{
+    ViewBuf ret;
+    bool ok = (op1 && op2) ? TRTC_Inner_Product(V(a), V(b), D(init), ret, F(op1), F(op2))
+                           : TRTC_Inner_Product(V(a), V(b), D(init), ret);
+    if (!ok) return nullptr;
+    return dv_from_viewbuf(ret, D(init).name_view_cls().c_str());
+}
+int n_all_of(void* v, void* p)
+{
+    bool r = false;
+    bool ok = TRTC_All_Of(V(v), F(p), r);
+    return bool_i(ok, r);
+}
+int n_any_of(void* v, void* p)
+{
+    bool r = false;
+    bool ok = TRTC_Any_Of(V(v), F(p), r);
+    return bool_i(ok, r);
+}
+int n_none_of(void* v, void* p)
+{
+    bool r = false;
+    bool ok = TRTC_None_Of(V(v), F(p), r);
+    return bool_i(ok, r);
+}
+int n_is_partitioned(void* v, void* p)
+{
+    bool r = false;
+    bool ok = TRTC_Is_Partitioned(V(v), F(p), r);
+    return bool_i(ok, r);
+}
+// not found: the vector's size (api_Searching.cpp:9-20)
+static long long found_ll(bool ok, size_t r, size_t n) { return !ok ? -1 : (r == (size_t)-1 ? (long long)n : (long long)r); }
+long long n_find(void* v, void* val)
+{
+    size_t r = 0;
+    bool ok = TRTC_Find(V(v), D(val), r);
+    return found_ll(ok, r, V(v).size());
+}
+long long n_find_if(void* v, void* p)
+{
+    size_t r = 0;
+    bool ok = TRTC_Find_If(V(v), F(p), r);
+    return found_ll(ok, r, V(v).size());
+}
+long long n_find_if_not(void* v, void* p)
+{
+    size_t r = 0;
+    bool ok = TRTC_Find_If_Not(V(v), F(p), r);
+    return found_ll(ok, r, V(v).size());
+}
+long long n_mismatch(void* a, void* b, void* p)
+{
+    size_t r = 0;
+    bool ok = p ? TRTC_Mismatch(V(a), V(b), F(p), r) : TRTC_Mismatch(V(a), V(b), r);
+    return found_ll(ok, r, V(a).size());
+}
+unsigned long long n_lower_bound(void* v, void* val, void* c)
+{
+    size_t r;
+    return (c ? TRTC_Lower_Bound(V(v), D(val), F(c), r) : TRTC_Lower_Bound(V(v), D(val), r)) ? r : ~0ull;
+}
+unsigned long long n_upper_bound(void* v, void* val, void* c)
+{
+    size_t r;
+    return (c ? TRTC_Upper_Bound(V(v), D(val), F(c), r) : TRTC_Upper_Bound(V(v), D(val), r)) ? r : ~0ull;
+}
+int n_binary_search(void* v, void* val, void* c)
+{
+    bool r = false;
+    bool ok = c ? TRTC_Binary_Search(V(v), D(val), F(c), r) : TRTC_Binary_Search(V(v), D(val), r);
+    return bool_i(ok, r);
+}
+int n_lower_bound_v(void* v, void* vals, void* res, void* c)
+{
+    return ok_i(c ? TRTC_Lower_Bound_V(V(v), V(vals), V(res), F(c)) : TRTC_Lower_Bound_V(V(v), V(vals), V(res)));
+}
+int n_upper_bound_v(void* v, void* vals, void* res, void* c)
+{
+    return ok_i(c ? TRTC_Upper_Bound_V(V(v), V(vals), V(res), F(c)) : TRTC_Upper_Bound_V(V(v), V(vals), V(res)));
+}
+int n_binary_search_v(void* v, void* vals, void* res, void* c)
+{
+    return ok_i(c ? TRTC_Binary_Search_V(V(v), V(vals), V(res), F(c)) : TRTC_Binary_Search_V(V(v), V(vals), V(res)));
+}
+unsigned long long n_partition_point(void* v, void* p)
+{
+    size_t r;
+    return TRTC_Partition_Point(V(v), F(p), r) ? r : ~0ull;
+}
+unsigned long long n_is_sorted_until(void* v, void* c)
+{
+    size_t r;
+    return (c ? TRTC_Is_Sorted_Until(V(v), F(c), r) : TRTC_Is_Sorted_Until(V(v), r)) ? r : ~0ull;
+}
+int n_merge(void* a, void* b, void* o, void* c)
+{
+    return ok_i(c ? TRTC_Merge(V(a), V(b), V(o), F(c)) : TRTC_Merge(V(a), V(b), V(o)));
+}
+int n_merge_by_key(void* k1, void* k2, void* v1, void* v2, void* ko, void* vo, void* c)
+{
+    return ok_i(c ? TRTC_Merge_By_Key(V(k1), V(k2), V(v1), V(v2), V(ko), V(vo), F(c))
+                  : TRTC_Merge_By_Key(V(k1), V(k2), V(v1), V(v2), V(ko), V(vo)));
+}
+#undef F
+#undef V
+#undef D
 
 int n_sort(void* ptr_vec, void* ptr_comp)
 {

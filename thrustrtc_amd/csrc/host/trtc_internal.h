@@ -160,6 +160,10 @@ bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, voi
 // nullptr otherwise.
 void* contiguous_data(const DVVectorLike& v);
 
+// The single-launch reduction of the primitive core over any view (init
+// first when given; an empty vector without init is an error).
+bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Functor& op, ViewBuf& ret);
+
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull);
 
 }  // namespace trtc_host

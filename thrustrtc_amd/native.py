@@ -102,6 +102,48 @@ SIGNATURES = {
     "n_dvtransform_create": (_vp, [_vp, _s, _vp]),
     "n_dvzipped_create": (_vp, [_vp, _vp]),
     "n_dvcustomvector_create": (_vp, [_vp, _vp, _s, _s, _s, _ull, _u]),
+    "n_dvtuple_create": (_vp, [_vp, _vp]),
+    "n_replace": (_i, [_vp, _vp, _vp]),
+    "n_replace_if": (_i, [_vp, _vp, _vp]),
+    "n_replace_copy": (_i, [_vp, _vp, _vp, _vp]),
+    "n_replace_copy_if": (_i, [_vp, _vp, _vp, _vp]),
+    "n_for_each": (_i, [_vp, _vp]),
+    "n_adjacent_difference": (_i, [_vp, _vp, _vp]),
+    "n_sequence": (_i, [_vp, _vp, _vp]),
+    "n_tabulate": (_i, [_vp, _vp]),
+    "n_transform": (_i, [_vp, _vp, _vp]),
+    "n_transform_binary": (_i, [_vp, _vp, _vp, _vp]),
+    "n_transform_if": (_i, [_vp, _vp, _vp, _vp]),
+    "n_transform_if_stencil": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "n_transform_binary_if_stencil": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "n_gather": (_i, [_vp, _vp, _vp]),
+    "n_gather_if": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "n_scatter": (_i, [_vp, _vp, _vp]),
+    "n_scatter_if": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "n_swap": (_i, [_vp, _vp]),
+    "n_equal": (_i, [_vp, _vp, _vp]),
+    "n_min_element": (_ull, [_vp, _vp]),
+    "n_max_element": (_ull, [_vp, _vp]),
+    "n_minmax_element": (_i, [_vp, _vp, ctypes.POINTER(_ull)]),
+    "n_inner_product": (_vp, [_vp, _vp, _vp, _vp, _vp]),
+    "n_all_of": (_i, [_vp, _vp]),
+    "n_any_of": (_i, [_vp, _vp]),
+    "n_none_of": (_i, [_vp, _vp]),
+    "n_is_partitioned": (_i, [_vp, _vp]),
+    "n_find": (_ll, [_vp, _vp]),
+    "n_find_if": (_ll, [_vp, _vp]),
+    "n_find_if_not": (_ll, [_vp, _vp]),
+    "n_mismatch": (_ll, [_vp, _vp, _vp]),
+    "n_lower_bound": (_ull, [_vp, _vp, _vp]),
+    "n_upper_bound": (_ull, [_vp, _vp, _vp]),
+    "n_binary_search": (_i, [_vp, _vp, _vp]),
+    "n_lower_bound_v": (_i, [_vp, _vp, _vp, _vp]),
+    "n_upper_bound_v": (_i, [_vp, _vp, _vp, _vp]),
+    "n_binary_search_v": (_i, [_vp, _vp, _vp, _vp]),
+    "n_partition_point": (_ull, [_vp, _vp]),
+    "n_is_sorted_until": (_ull, [_vp, _vp]),
+    "n_merge": (_i, [_vp, _vp, _vp, _vp]),
+    "n_merge_by_key": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "n_copy_if": (_u, [_vp, _vp, _vp]),
     "n_copy_if_stencil": (_u, [_vp, _vp, _vp, _vp]),
     "n_sort": (_i, [_vp, _vp]),
