@@ -4,6 +4,7 @@ and its host logic (functor code generation, struct layout, hipRTC
 instantiation of the kernel templates for gfx950) works without a device.
 No compute call is made here."""
 import ctypes
+import json
 import os
 import re
 import struct
@@ -35,8 +36,17 @@ def test_every_declared_symbol_is_exported(nm):
     assert set(syms) == set(nm.SIGNATURES), set(syms) ^ set(nm.SIGNATURES)
 
 
+def test_every_reference_function_declared_and_exported(nm):
+    # every function of the reference's python/api.h, same name
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "api_h_functions.json")))["functions"]
+    assert len(ref) == 149
+    syms = set(header_symbols())
+    assert not [f for f in ref if f not in syms]
+    assert not [f for f in ref if not hasattr(nm.native, f)]
+
+
 def test_reference_hot_path_subset_present():
-    # the reference's python/api.h names on the hot path (SURVEY scope table b)
+    # the reference's python/api.h names of the primitive core
     need = ["n_sort", "n_sort_by_key", "n_inclusive_scan", "n_exclusive_scan", "n_inclusive_scan_by_key",
             "n_exclusive_scan_by_key", "n_reduce", "n_copy_if", "n_copy_if_stencil", "n_trtc_try_init",
             "n_set_verbose", "n_set_kernel_debug", "n_wait", "n_get_ptx_arch", "n_dvvector_create",
