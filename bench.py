@@ -246,7 +246,29 @@ def leg_consumers(trtc, native, wl, a, log):
         return (o1.to_host() == orc.transform_scan(x, "int32_t", "Negate")).all()
     entry("transform_inclusive_scan_i32_negate", lambda: trtc.Transform_Inclusive_Scan(dx, o1, trtc.Negate(),
           trtc.Plus()), "trtc_scan_k<", lambda _: 8 * n, chk_tscan)
-    del dx, o1, o2
+    # the other algorithms: element-wise, random gather, index reduction, merge
+    def chk_neg(_):
+        return (o1.to_host() == -x).all()
+    entry("transform_negate_i32", lambda: trtc.Transform(dx, o1, trtc.Negate()), "trtc_for", lambda _: 8 * n, chk_neg)
+    gmap = (wl.u32_uniform(n, 53) % np.uint32(n)).astype(np.int32)
+    dmap = trtc.device_vector_from_numpy(gmap)
+
+    def chk_gather(_):
+        return (o1.to_host() == x[gmap]).all()
+    entry("gather_i32_random_map", lambda: trtc.Gather(dmap, dx, o1), "trtc_for", lambda _: 12 * n, chk_gather)
+    del dmap
+
+    def chk_min(r):
+        return r == int(np.argmin(x))
+    entry("min_element_i32", lambda: trtc.Min_Element(dx), "trtc_reduce_k<", lambda _: 4 * n, chk_min)
+    half = n // 2
+    sa, sb = np.sort(x[:half]), np.sort(x[half:])
+    da, db = trtc.device_vector_from_numpy(sa), trtc.device_vector_from_numpy(sb)
+
+    def chk_merge(_):
+        return (o1.to_host() == np.sort(x)).all()
+    entry("merge_i32_2x2p27", lambda: trtc.Merge(da, db, o1), "trtc_merge_k<", lambda _: 8 * n, chk_merge)
+    del da, db, dx, o1, o2
 
     # comparison sort with a user comparator: LDS tile sort + merge passes
     ns = 1 << 24
