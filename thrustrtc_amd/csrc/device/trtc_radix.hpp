@@ -287,13 +287,17 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const unsigned d = dig[k];
-            unsigned long long peers = ~0ull;
+            // lanes whose digit differs from this lane's in some bit: per bit,
+            // the ballot XOR this lane's bit spread to all 64 bits (0 or ~0)
+            unsigned dlo = 0u, dhi = 0u;
 #pragma unroll
             for (int b = 0; b < 8; b++) {
-                const bool bit = (d >> b) & 1u;
-                const unsigned long long bal = ballot(bit);
-                peers &= bit ? bal : ~bal;
+                const unsigned m = (unsigned)(((int)(d << (31 - b))) >> 31);
+                const unsigned long long bal = ballot(m != 0u);
+                dlo |= (unsigned)bal ^ m;
+                dhi |= (unsigned)(bal >> 32) ^ m;
             }
+            const unsigned long long peers = ~(((unsigned long long)dhi << 32) | dlo);
             const unsigned leader = (unsigned)__builtin_ctzll(peers);
             const unsigned cnt = (unsigned)__builtin_popcountll(peers);
             unsigned old = 0;
