@@ -104,8 +104,7 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
         typedef unsigned v4u __attribute__((ext_vector_type(4)));
         const v4u* kv = (const v4u*)keys;
         const size_t nvec = a.n / PER_VEC;
-        for (size_t v = gid; v < nvec; v += stride) {
-            v4u x = __builtin_nontemporal_load(kv + v);
+        auto count = [&](const v4u& x) {
             U ks[PER_VEC];
             __builtin_memcpy(ks, &x, 16);
 #pragma unroll
@@ -116,7 +115,19 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
                     __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
             }
+        };
+        // four independent 16-byte loads in flight per thread before the
+        // counting (one at a time leaves the read latency-bound)
+        constexpr int UNR = 4;
+        size_t v = gid;
+        for (; v + (UNR - 1) * stride < nvec; v += UNR * stride) {
+            v4u x[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) x[u] = __builtin_nontemporal_load(kv + v + u * stride);
+#pragma unroll
+            for (int u = 0; u < UNR; u++) count(x[u]);
         }
+        for (; v < nvec; v += stride) count(__builtin_nontemporal_load(kv + v));
         done_keys = nvec * PER_VEC;
     }
     for (size_t i = done_keys + gid; i < a.n; i += stride) {
@@ -126,7 +137,19 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
             __hip_atomic_fetch_add(&s_h[wave][p][(k >> (8 * p)) & 255], 1u, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    for (size_t i = gid; i < a.zero_words; i += stride) a.zero_ptr[i] = 0;
+    {
+        // clear the passes' lookback state, 16 bytes per store when aligned
+        size_t z0 = 0;
+        if ((((size_t)a.zero_ptr) & 15) == 0) {
+            typedef unsigned v4z __attribute__((ext_vector_type(4)));
+            v4z* zp = (v4z*)a.zero_ptr;
+            const size_t nz = a.zero_words / 4;
+            const v4z zero = {0u, 0u, 0u, 0u};
+            for (size_t i = gid; i < nz; i += stride) zp[i] = zero;
+            z0 = nz * 4;
+        }
+        for (size_t i = z0 + gid; i < a.zero_words; i += stride) a.zero_ptr[i] = 0;
+    }
     __syncthreads();
 
     for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
