@@ -29,10 +29,10 @@
 namespace trtc {
 
 template <int KB> struct KeyWord;
-template <> struct KeyWord<1> { typedef uint8_t U; };
-template <> struct KeyWord<2> { typedef uint16_t U; };
-template <> struct KeyWord<4> { typedef uint32_t U; };
-template <> struct KeyWord<8> { typedef uint64_t U; };
+template <> struct KeyWord<1> { typedef uint8_t U; typedef int8_t S; };
+template <> struct KeyWord<2> { typedef uint16_t U; typedef int16_t S; };
+template <> struct KeyWord<4> { typedef uint32_t U; typedef int32_t S; };
+template <> struct KeyWord<8> { typedef uint64_t U; typedef int64_t S; };
 
 template <int VB> struct ValWord;
 template <> struct ValWord<0> { typedef uint8_t U; };
@@ -43,14 +43,21 @@ template <> struct ValWord<8> { typedef uint64_t U; };
 
 enum KeyKind { kUnsigned = 0, kSigned = 1, kFloat = 2 };
 
+// Order-preserving unsigned image of a key's bits:
+//   unsigned: bits;  signed: bits ^ SIGN;
+//   float: non-negative -> bits | SIGN, negative -> SIGN - magnitude = -bits
+//   (so -0.0 and +0.0 map to the same image and keep their input order).
+// Written without branches, (bits ^ (s | xm)) - s with s the float sign mask,
+// so the per-key loops stay one basic block and their LDS reads can be
+// issued together.
 template <class U> TRTC_DEV U sort_key(U bits, int kind, U flip)
 {
+    typedef typename KeyWord<(int)sizeof(U)>::S SU;
     constexpr U SIGN = (U)((U)1 << (sizeof(U) * 8 - 1));
-    U k;
-    if (kind == kUnsigned) k = bits;
-    else if (kind == kSigned) k = (U)(bits ^ SIGN);
-    else k = (bits & SIGN) ? (U)(SIGN - (U)(bits & (U)~SIGN)) : (U)(bits | SIGN);
-    return (U)(k ^ flip);
+    const U fm = kind == kFloat ? (U)~(U)0 : (U)0;
+    const U xm = kind == kUnsigned ? (U)0 : SIGN;
+    const U s = (U)((U)((SU)bits >> (sizeof(U) * 8 - 1)) & fm);
+    return (U)((U)((U)(bits ^ (U)(s | xm)) - s) ^ flip);
 }
 
 // Status words of the digit lookback: two flag bits on top of a count.
@@ -229,7 +236,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     typedef typename ValWord<VB>::U V;
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
-    constexpr int STAGE_BYTES = TILE * (KB > VB ? KB : VB);
+    constexpr int VSTAGE_BYTES = VB ? TILE * VB : 16;
     constexpr int LBB = 4;
     typedef StatusBits<S> SB;
     typedef S P;  // positions fit the status count field
@@ -240,9 +247,12 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     __shared__ P s_base[256];                     // global position of local slot 0 per digit
     __shared__ unsigned s_dscan[4];
     __shared__ unsigned s_tile;
-    __shared__ __attribute__((aligned(16))) unsigned char s_stage[STAGE_BYTES];
-    U* s_keys = (U*)s_stage;
-    V* s_vals = (V*)s_stage;
+    // keys and values are staged in separate buffers, both before the
+    // lookback, so the write-out needs no further barrier
+    __shared__ __attribute__((aligned(16))) unsigned char s_kstage[TILE * KB];
+    __shared__ __attribute__((aligned(16))) unsigned char s_vstage[VSTAGE_BYTES];
+    U* s_keys = (U*)s_kstage;
+    V* s_vals = (V*)s_vstage;
 
     for (int i = threadIdx.x; i < WAVES * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < 256; i += BLOCK) s_hist[i] = 0;
@@ -362,11 +372,11 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
         for (int ww = 0; ww < WAVES; ww++) s_wcnt[ww][d] += dstart;
     }
     lds_barrier();
-    unsigned lpos[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        lpos[k] = s_wcnt[wave][dig[k] & 255u] + (dig[k] >> 8);
-        s_keys[lpos[k]] = key[k];
+        const unsigned lpos = s_wcnt[wave][dig[k] & 255u] + (dig[k] >> 8);
+        s_keys[lpos] = key[k];
+        if (VB) s_vals[lpos] = val[k];
     }
 
     // 5. finish the lookback for digit d
@@ -407,29 +417,35 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     }
     lds_barrier();
 
-    // 6. write each digit run to its global slot
+    // 6. write each digit run to its global slot.  All LDS reads are issued
+    //    before the first store (separate loops), so they overlap instead of
+    //    waiting one key at a time.
     U* kout = job.kout;
+    U kb[ITEMS];
     P gpos[ITEMS];
 #pragma unroll
+    for (int r = 0; r < ITEMS; r++) kb[r] = s_keys[(unsigned)(r * BLOCK) + threadIdx.x];
+#pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
-        U kb = s_keys[j];
-        unsigned d = (unsigned)(sort_key<U>(kb, kind, flip) >> shift) & 255u;
-        gpos[r] = s_base[d] + j;
-        // gpos < n holds by construction; the check keeps a logic error from
-        // ever writing outside the output
-        if ((full || j < valid) && gpos[r] < (P)n) kout[gpos[r]] = kb;
+        const unsigned d = (unsigned)(sort_key<U>(kb[r], kind, flip) >> shift) & 255u;
+        gpos[r] = s_base[d] + (unsigned)(r * BLOCK) + threadIdx.x;
+    }
+    // gpos < n holds by construction; the check keeps a logic error from
+    // ever writing outside the output
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
+        if ((full || j < valid) && gpos[r] < (P)n) kout[gpos[r]] = kb[r];
     }
     if (VB) {
-        lds_barrier();
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) s_vals[lpos[k]] = val[k];
-        lds_barrier();
         V* vout = job.vout;
+        V vb[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) vb[r] = s_vals[(unsigned)(r * BLOCK) + threadIdx.x];
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
-            unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
-            if ((full || j < valid) && gpos[r] < (P)n) vout[gpos[r]] = s_vals[j];
+            const unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
+            if ((full || j < valid) && gpos[r] < (P)n) vout[gpos[r]] = vb[r];
         }
     }
 }
