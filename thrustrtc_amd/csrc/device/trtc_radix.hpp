@@ -314,30 +314,42 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     }
 
     // 3. in-wave ranking: peers = lanes with the same digit in this slot;
-    //    the rank is packed above the digit (rank < 64*ITEMS)
+    //    the rank is packed above the digit (rank < 64*ITEMS).  The lowest
+    //    peer adds the peer count to the wave's digit counter and every peer
+    //    reads the counter back right after: a wave's LDS operations execute
+    //    in issue order, so the read sees this slot's add and no later one.
     {
-        const unsigned long long lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const unsigned d = dig[k];
-            // lanes whose digit differs from this lane's in some bit: per bit,
-            // the ballot XOR this lane's bit spread to all 64 bits (0 or ~0)
-            unsigned dlo = 0u, dhi = 0u;
+            // lanes whose digit equals this lane's in every bit: per bit, keep
+            // the lanes where the ballot agrees with this lane's bit spread to
+            // all 32 bits (m = 0 or ~0): acc & ~(ballot ^ m), one v_bitop3
+            // (truth table 0x90) per half.  The empty asm hides that m is a bit
+            // field, so the ballot stays one compare on m rather than a second
+            // shift-and-compare of the digit.  All eight ballots come first, so
+            // no bitop3 waits on the compare just before it.
+            int m[8];
+            unsigned long long bal[8];
 #pragma unroll
             for (int b = 0; b < 8; b++) {
-                const unsigned m = (unsigned)(((int)(d << (31 - b))) >> 31);
-                const unsigned long long bal = ballot(m != 0u);
-                dlo |= (unsigned)bal ^ m;
-                dhi |= (unsigned)(bal >> 32) ^ m;
+                m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
+                __asm__("" : "+v"(m[b]));
+                bal[b] = ballot(m[b] != 0);
             }
-            const unsigned long long peers = ~(((unsigned long long)dhi << 32) | dlo);
-            const unsigned leader = (unsigned)__builtin_ctzll(peers);
-            const unsigned cnt = (unsigned)__builtin_popcountll(peers);
-            unsigned old = 0;
-            if (lane == leader)
-                old = __hip_atomic_fetch_add(&s_wcnt[wave][d], cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            old = (unsigned)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
-            dig[k] |= (old + (unsigned)__builtin_popcountll(peers & lt_mask)) << 8;
+            unsigned plo = ~0u, phi = ~0u;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
+                phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
+            }
+            // peers in lower lanes, and all peers
+            const unsigned below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+            const unsigned cnt = (unsigned)__builtin_popcount(plo) + (unsigned)__builtin_popcount(phi);
+            unsigned* ctr = &s_wcnt[wave][d];
+            if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            dig[k] |= (after - cnt + below) << 8;
         }
     }
     lds_barrier();
