@@ -42,7 +42,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     T* lds = (T*)s_raw[wave];
     ST* ldss = (ST*)s_raw[wave];
 
-    // contiguous aligned views: 16-byte vector loads of each lane's run
+    // contiguous aligned views: striped 16-byte vector loads
     constexpr bool DVI = DirectVec<In, T, ITEMS>::value;
     constexpr bool DVS = DirectVec<Sten, ST, ITEMS>::value;
     bool direct_in = false, direct_st = false;
@@ -51,7 +51,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     T x[ITEMS];
     int nvalid = ITEMS;
     if constexpr (DVI) {
-        if (direct_in) load_run_vec<ITEMS>(Contig<In>::ptr(in) + wave_base + (size_t)lane * ITEMS, x);
+        if (direct_in) load_striped_vec<ITEMS>(Contig<In>::ptr(in) + wave_base, lds, x);
     }
     if (!direct_in) {
 #pragma unroll
@@ -73,7 +73,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     if constexpr (HAS_ST) {
         ST sv[ITEMS];
         if constexpr (DVS) {
-            if (direct_st) load_run_vec<ITEMS>(Contig<Sten>::ptr(sten) + wave_base + (size_t)lane * ITEMS, sv);
+            if (direct_st) load_striped_vec<ITEMS>(Contig<Sten>::ptr(sten) + wave_base, ldss, sv);
         }
         if (!direct_st) {
             wave_sync();
@@ -150,9 +150,12 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
         if (P + i < cap) out[P + i] = (OutT)pack[i];
 }
 
+// Items per thread of the compaction tiles (copy_if, unique, partition,
+// reduce_by_key; 512 threads): 128 bytes per lane for 4- and 8-byte elements.
+// The host's tile size follows the same table.
 template <class T> struct CopyIfItems
 {
-    static constexpr int value = sizeof(T) <= 4 ? 16 : (sizeof(T) <= 8 ? 8 : 4);
+    static constexpr int value = sizeof(T) <= 4 ? 32 : (sizeof(T) <= 8 ? 16 : 4);
 };
 
 }  // namespace trtc

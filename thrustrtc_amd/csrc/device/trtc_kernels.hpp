@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_reduce_by_key_k(Keys keys, Vals vals, KOut kout, VOut vout, Pred pred, Op op, trtc::CopyIfArgs a)
 {
     typedef typename Keys::value_t K;
-    constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::ScanByKeyItems<K>::value : trtc::ScanByKeyItems<V>::value;
+    constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::CopyIfItems<K>::value : trtc::CopyIfItems<V>::value;
     trtc::reduce_by_key_tile<TRTC_SCAN_BLOCK, ITEMS, V, K>(keys, vals, kout, vout, pred, op, a);
 }
 
