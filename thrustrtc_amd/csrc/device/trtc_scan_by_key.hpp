@@ -43,8 +43,10 @@ TRTC_DEV void scan_by_key_tile(Keys& keys, Vals& vals, Out& out, Pred& pred, Op&
     constexpr int WELEMS = 64 * ITEMS;
     constexpr size_t TILE = (size_t)BLOCK * ITEMS;
     typedef Seg<V> S;
-    __shared__ V s_v[WAVES][WaveTile<ITEMS>::padded];
-    __shared__ K s_k[WAVES][WaveTile<ITEMS>::padded];
+    // one transpose buffer per wave, used for the keys, the values, then the
+    // outputs (each load syncs the wave before it overwrites the buffer)
+    constexpr int ESZ = sizeof(V) > sizeof(K) ? sizeof(V) : sizeof(K);
+    __shared__ __attribute__((aligned(16))) unsigned char s_raw[WAVES][WaveTile<ITEMS>::padded * ESZ];
     __shared__ S s_wtot[WAVES];
     __shared__ int s_wvalid[WAVES];
     __shared__ S s_prefix;
@@ -59,8 +61,8 @@ TRTC_DEV void scan_by_key_tile(Keys& keys, Vals& vals, Out& out, Pred& pred, Op&
     const size_t tile_base = tile * TILE;
     const size_t wave_base = tile_base + (size_t)wave * WELEMS;
     const bool full = tile_base + TILE <= n;
-    V* ldv = s_v[wave];
-    K* ldk = s_k[wave];
+    V* ldv = (V*)s_raw[wave];
+    K* ldk = (K*)s_raw[wave];
     SegOp<V, Op> sop{op};
 
     // blocked loads (16-byte vector loads for contiguous views); the key
@@ -179,9 +181,11 @@ TRTC_DEV void scan_by_key_tile(Keys& keys, Vals& vals, Out& out, Pred& pred, Op&
     }
 }
 
+// Items per thread of the by-key scan (512 threads): 128 bytes per lane for
+// 4- and 8-byte elements, as the compaction tiles.
 template <class V> struct ScanByKeyItems
 {
-    static constexpr int value = sizeof(V) <= 4 ? 16 : (sizeof(V) <= 8 ? 8 : 4);
+    static constexpr int value = sizeof(V) <= 4 ? 32 : (sizeof(V) <= 8 ? 16 : 4);
 };
 
 }  // namespace trtc

@@ -29,13 +29,12 @@ constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: scans, selections, by-
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Items per thread of the scan-family tiles (must match ReduceItems /
-// ScanByKeyItems in the device headers).
+// Items per thread of the reduce chunks (must match ReduceItems).
 int items_for(size_t elem_size) { return elem_size <= 4 ? 16 : (elem_size <= 8 ? 8 : 4); }
-// The whole-input scan's and the compaction tiles' items per thread (must
-// match ScanItems and CopyIfItems).
-int scan_items_for(size_t elem_size) { return elem_size <= 4 ? 32 : (elem_size <= 8 ? 16 : 4); }
-int sel_items_for(size_t elem_size) { return elem_size <= 4 ? 32 : (elem_size <= 8 ? 16 : 4); }
+// Items per thread of the single-pass tiles -- whole-input scan, compaction,
+// by-key scan and reduction: 128 bytes per lane (must match ScanItems,
+// CopyIfItems and ScanByKeyItems in the device headers).
+int tile_items_for(size_t elem_size) { return elem_size <= 4 ? 32 : (elem_size <= 8 ? 16 : 4); }
 size_t words_for(size_t bytes) { return (bytes + 3) / 4; }
 
 std::string vcls(const DeviceViewable& v) { return canon_type(v.name_view_cls()); }
@@ -379,7 +378,7 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kScanBlock * scan_items_for(tsz);
+    size_t tile = (size_t)kScanBlock * tile_items_for(tsz);
     size_t tiles = (n + tile - 1) / tile;
     LookbackScratch lb;
     if (!lb.make(tiles, tsz, s)) return false;
@@ -502,7 +501,7 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
     stencil_arg(args, sten);
     args.view(vout);
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * sel_items_for(vin.elem_size()), kScanBlock, 8, sten != nullptr);
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(vin.elem_size()), kScanBlock, 8, sten != nullptr);
 }
 
 // Unique / Unique_By_Key (unique.cpp:5-95): keep item i when i == 0 or
@@ -534,7 +533,7 @@ uint32_t general_unique(const DVVectorLike& kin, const DVVectorLike* vin, DVVect
         args.view(kout);
     }
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * sel_items_for(esz), kScanBlock, 8, false);
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(esz), kScanBlock, 8, false);
 }
 
 // Stable two-way split (partition.cpp:28-130): items whose predicate holds go
@@ -557,7 +556,7 @@ uint32_t general_partition_copy(const DVVectorLike& vin, const DVVectorLike* ste
     args.view(tout);
     args.view(fout);
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * sel_items_for(vin.elem_size()), kScanBlock, 8,
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(vin.elem_size()), kScanBlock, 8,
                          sten != nullptr);
 }
 
@@ -609,7 +608,7 @@ uint32_t general_reduce_by_key(const DVVectorLike& kin, const DVVectorLike& vin,
     args.view(pred);
     args.view(op);
     // state {V v; unsigned flag; unsigned tails;} fits align_up(V, 8) + 8 bytes
-    return run_selection(name, args, n, (size_t)kScanBlock * sel_items_for(std::max(kin.elem_size(), vsz)), kScanBlock,
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(std::max(kin.elem_size(), vsz)), kScanBlock,
                          align_up(vsz, 8) + 8, false);
 }
 
@@ -634,7 +633,7 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kScanBlock * items_for(std::max(keys.elem_size(), vsz));
+    size_t tile = (size_t)kScanBlock * tile_items_for(std::max(keys.elem_size(), vsz));
     size_t tiles = (n + tile - 1) / tile;
     size_t seg_bytes = align_up(vsz + 4, std::min<size_t>(std::max<size_t>(vsz, 4), 8));
     LookbackScratch lb;
