@@ -119,3 +119,47 @@ def test_reduce_f32_2p29(trtc, wl):
     del f
     r = float(trtc.Reduce(d))
     assert abs(r - g["sum_fp64"]) <= 1e-5 * abs(g["sum_fp64"])
+
+
+# Sizes from 2^25 keys (4-byte pairs) use the 1024-thread digit-pass tiles;
+# the 2^28 cases above cover u32 and f32 keys with u32 values.  These cover the
+# other key / value widths, keys-only and descending order on those tiles, with
+# a partial last tile, against numpy's stable sort (the order std::stable_sort
+# gives: -0.0 and +0.0 compare equal; descending = stable ascending sort of an
+# order-reversing map of the key).
+LARGE_N = (1 << 25) + 12345
+
+
+@pytest.mark.parametrize("kdt,vdt,greater", [
+    (np.uint64, np.uint32, False),
+    (np.int32, None, True),
+    (np.float64, np.uint64, False),
+    (np.int16, np.uint32, True),
+    (np.float32, np.uint8, True),
+])
+def test_sort_large_tiles(trtc, kdt, vdt, greater):
+    n = LARGE_N
+    say(f"large-tile sort {np.dtype(kdt).name} / {None if vdt is None else np.dtype(vdt).name} "
+        f"{'Greater' if greater else 'Less'} n={n}")
+    rng = np.random.default_rng(7)
+    if np.dtype(kdt).kind == "f":
+        k = rng.standard_normal(n).astype(kdt)
+        k[rng.random(n) < 0.01] = -0.0
+        k[rng.random(n) < 0.01] = 0.0
+        order_key = -k if greater else k
+    else:
+        info = np.iinfo(kdt)
+        k = rng.integers(info.min, info.max, size=n, dtype=kdt, endpoint=True)
+        order_key = ~k if greater else k
+    perm = np.argsort(order_key, kind="stable")
+    dk = trtc.device_vector_from_numpy(k)
+    comp = trtc.Greater() if greater else None
+    if vdt is None:
+        trtc.Sort(dk, comp)
+        assert np.array_equal(dk.to_host().view(np.uint8), k[perm].view(np.uint8))
+    else:
+        v = (np.arange(n) % np.iinfo(vdt).max).astype(vdt) if np.dtype(vdt).itemsize < 4 else np.arange(n, dtype=vdt)
+        dv = trtc.device_vector_from_numpy(v)
+        trtc.Sort_By_Key(dk, dv, comp)
+        assert np.array_equal(dk.to_host().view(np.uint8), k[perm].view(np.uint8))
+        assert np.array_equal(dv.to_host(), v[perm])

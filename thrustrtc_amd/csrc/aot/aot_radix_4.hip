@@ -5,8 +5,9 @@
 namespace {
 template <int VB, class S> void reg_pass(const char* sn)
 {
-    std::string name = std::string("trtc_radix_pass_k<4, ") + std::to_string(VB) + ", " + sn + ">";
-    TRTC_REG(name.c_str(), (&trtc_radix_pass_k<4, VB, S>));
+    std::string name = std::string("trtc_radix_pass_k<4, ") + std::to_string(VB) + ", " + sn + ", ";
+    TRTC_REG((name + "512>").c_str(), (&trtc_radix_pass_k<4, VB, S, 512>));
+    TRTC_REG((name + "1024>").c_str(), (&trtc_radix_pass_k<4, VB, S, 1024>));
 }
 struct RegRadix
 {

@@ -124,18 +124,21 @@ __global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_hist_k(trtc::RadixHistA
     trtc::radix_hist<TRTC_BLOCK, KB>(a);
 }
 
-// Radix digit pass: 512 threads; 16 keys per thread for keys and values of
-// up to 4 bytes (8192-key tiles), 8 per thread when either is 8 bytes.
-#define TRTC_RADIX_BLOCK 512
+// Radix digit pass: BLOCK = 512 or 1024 threads; 16 keys per thread for keys
+// and values of up to 4 bytes, 8 per thread when either is 8 bytes.  Large
+// sorts use 1024 threads (16384-key tiles for 4-byte pairs, one workgroup per
+// CU at 150 KB LDS): twice as long digit runs per tile, 2^28 u32/u32 pass
+// 1.341 -> 1.291 ms against 512 threads; smaller sorts keep 512 threads for
+// more tiles per CU.
 template <int KB, int VB> struct RadixItems
 {
     static constexpr int value = (KB > 4 || VB > 4) ? 8 : 16;
 };
 
-template <int KB, int VB, class S>
-__global__ void __launch_bounds__(TRTC_RADIX_BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a)
+template <int KB, int VB, class S, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a)
 {
-    trtc::radix_pass<TRTC_RADIX_BLOCK, RadixItems<KB, VB>::value, KB, VB, S>(a);
+    trtc::radix_pass<BLOCK, RadixItems<KB, VB>::value, KB, VB, S>(a);
 }
 
 // ---- batched segments (one launch over many independent sequences) ----

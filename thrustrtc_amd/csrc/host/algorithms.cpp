@@ -124,9 +124,11 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         vals_src = vals;
     }
     const int KB = key_bytes;
-    // tile shape of trtc_radix_pass_k (RadixItems in the device header)
-    const unsigned kRadixBlock = 512;
+    // tile shape of trtc_radix_pass_k (RadixItems in the device header):
+    // 1024 threads while that still gives at least 8 tiles per CU
     const int ITEMS = (KB > 4 || val_bytes > 4) ? 8 : 16;
+    const unsigned kRadixBlock =
+        n / (1024 * (size_t)ITEMS) >= 8 * (size_t)std::max(1, context().num_cus) ? 1024u : 512u;
     const size_t TILE = (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
     const bool wide = n >= (1ull << 30);
@@ -170,7 +172,7 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
 
     KernelRef pk;
     std::string pname = "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
-                        (wide ? "unsigned long long" : "unsigned int") + ">";
+                        (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ">";
     if (!get_template_kernel(pname, "", pk)) return false;
     trtc::RadixPassArgs pa;
     void* kbuf[2] = {keys, ktmp.p};
