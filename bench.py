@@ -670,7 +670,7 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "radix_pass_traffic.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        if t.get("n") == N and pass_name.startswith(t.get("kernel", "?")[:-1]):
+        if t.get("n") == N and pass_name == t.get("kernel"):
             traffic = round(t["read_bytes"] + t["write_bytes"])
 
     # the per-GPU shape of an 8-way sharded 2^28 sort: 2^25 pairs
