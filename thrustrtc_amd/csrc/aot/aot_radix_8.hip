@@ -5,9 +5,12 @@
 namespace {
 template <int VB, class S> void reg_pass(const char* sn)
 {
+    // 512 threads: key kind read at run time; 1024 threads: one per key kind
     std::string name = std::string("trtc_radix_pass_k<8, ") + std::to_string(VB) + ", " + sn + ", ";
-    TRTC_REG((name + "512>").c_str(), (&trtc_radix_pass_k<8, VB, S, 512>));
-    TRTC_REG((name + "1024>").c_str(), (&trtc_radix_pass_k<8, VB, S, 1024>));
+    TRTC_REG((name + "512, -1>").c_str(), (&trtc_radix_pass_k<8, VB, S, 512, -1>));
+    TRTC_REG((name + "1024, 0>").c_str(), (&trtc_radix_pass_k<8, VB, S, 1024, 0>));
+    TRTC_REG((name + "1024, 1>").c_str(), (&trtc_radix_pass_k<8, VB, S, 1024, 1>));
+    TRTC_REG((name + "1024, 2>").c_str(), (&trtc_radix_pass_k<8, VB, S, 1024, 2>));
 }
 struct RegRadix
 {

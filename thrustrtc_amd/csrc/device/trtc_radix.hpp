@@ -60,6 +60,17 @@ template <class U> TRTC_DEV U sort_key(U bits, int kind, U flip)
     return (U)((U)((U)(bits ^ (U)(s | xm)) - s) ^ flip);
 }
 
+// sort_key with the key kind fixed at compile time (KIND >= 0: the mapping
+// costs at most one xor for unsigned keys), or read at run time (KIND < 0).
+template <int KIND, class U> TRTC_DEV U sort_key_k(U bits, int kind, U flip)
+{
+    constexpr U SIGN = (U)((U)1 << (sizeof(U) * 8 - 1));
+    if constexpr (KIND == kUnsigned) return (U)(bits ^ flip);
+    else if constexpr (KIND == kSigned) return (U)(bits ^ (U)(SIGN ^ flip));
+    else if constexpr (KIND == kFloat) return sort_key<U>(bits, kFloat, flip);
+    else return sort_key<U>(bits, kind, flip);
+}
+
 // Status words of the digit lookback: two flag bits on top of a count.
 template <class S> struct StatusBits
 {
@@ -245,7 +256,7 @@ template <class U, class V, class S> struct RadixJob
     const unsigned* dstart32;             // [256] (null both: one-tile sequence)
 };
 
-template <int BLOCK, int ITEMS, int KB, int VB, class S, class MakeJob>
+template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND, class MakeJob>
 TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, int shift, int kind, int descending)
 {
     typedef typename KeyWord<KB>::U U;
@@ -312,7 +323,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-        U sk = (full || p < valid) ? sort_key<U>(key[k], kind, flip) : (U)~(U)0;
+        U sk = (full || p < valid) ? sort_key_k<KIND, U>(key[k], kind, flip) : (U)~(U)0;
         dig[k] = (unsigned)(sk >> shift) & 255u;
     }
     S* st = job.status;
@@ -449,7 +460,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     for (int r = 0; r < ITEMS; r++) kb[r] = s_keys[(unsigned)(r * BLOCK) + threadIdx.x];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
-        const unsigned d = (unsigned)(sort_key<U>(kb[r], kind, flip) >> shift) & 255u;
+        const unsigned d = (unsigned)(sort_key_k<KIND, U>(kb[r], kind, flip) >> shift) & 255u;
         gpos[r] = s_base[d] + (unsigned)(r * BLOCK) + threadIdx.x;
     }
     // gpos < n holds by construction; the check keeps a logic error from
@@ -474,7 +485,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
 
 
 // One digit pass over the whole input.
-template <int BLOCK, int ITEMS, int KB, int VB, class S>
+template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND = -1>
 TRTC_DEV void radix_pass(const RadixPassArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
@@ -492,7 +503,7 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
         j.dstart32 = nullptr;
         return j;
     };
-    radix_pass_core<BLOCK, ITEMS, KB, VB, S>(a.tile_counter, make, a.shift, a.kind, a.descending);
+    radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
 // One digit pass over a batch of segments.  Pass 0 reads the segment's
@@ -527,7 +538,7 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         }
         return j;
     };
-    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned>(a.tile_counter, make, a.shift, a.kind, a.descending);
+    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned, -1>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per

@@ -172,7 +172,8 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
 
     KernelRef pk;
     std::string pname = "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
-                        (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ">";
+                        (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ", " +
+                        std::to_string(kRadixBlock == 1024 ? kind : -1) + ">";
     if (!get_template_kernel(pname, "", pk)) return false;
     trtc::RadixPassArgs pa;
     void* kbuf[2] = {keys, ktmp.p};
