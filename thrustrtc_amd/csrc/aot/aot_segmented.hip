@@ -32,16 +32,20 @@ struct RegSeg
     RegSeg()
     {
         // 4-byte keys: keys only and 4-byte values
+        reg_seg_pass<4, 4, 1024, 16>();
         reg_seg_pass<4, 4, 512, 16>();
         reg_seg_pass<4, 4, 512, 8>();
         reg_seg_pass<4, 4, 256, 16>();
         reg_seg_pass<4, 4, 256, 8>();
+        reg_seg_pass<4, 0, 1024, 16>();
         reg_seg_pass<4, 0, 512, 16>();
         reg_seg_pass<4, 0, 256, 8>();
         reg_seg_hist<4>();
         // 8-byte keys with 8-byte or no values
+        reg_seg_pass<8, 8, 1024, 8>();
         reg_seg_pass<8, 8, 512, 8>();
         reg_seg_pass<8, 8, 256, 8>();
+        reg_seg_pass<8, 0, 1024, 8>();
         reg_seg_pass<8, 0, 512, 8>();
         reg_seg_pass<8, 0, 256, 8>();
         reg_seg_hist<8>();

@@ -360,15 +360,23 @@ std::string signature(const std::vector<SortSeg>& segs, int KB, int VB)
     return sig;
 }
 
+// Segments above 2^18 keys may take 1024-thread tiles (16384 keys for 4-byte
+// pairs, as the single large sort uses); the tuner keeps them only where
+// they win.
 std::vector<SortSchedule> candidates(int KB, int VB)
 {
     if (KB > 4 || VB > 4)
-        return {{{kNoLimit, 512, 8}}, {{kNoLimit, 256, 8}}, {{2048, 256, 8}, {kNoLimit, 512, 8}}};
+        return {{{kNoLimit, 512, 8}},
+                {{kNoLimit, 256, 8}},
+                {{2048, 256, 8}, {kNoLimit, 512, 8}},
+                {{2048, 256, 8}, {1u << 18, 512, 8}, {kNoLimit, 1024, 8}}};
     return {{{kNoLimit, 512, 16}},
             {{kNoLimit, 512, 8}},
             {{kNoLimit, 256, 16}},
             {{2048, 256, 8}, {kNoLimit, 512, 16}},
-            {{4096, 256, 16}, {kNoLimit, 512, 16}}};
+            {{4096, 256, 16}, {kNoLimit, 512, 16}},
+            {{2048, 256, 8}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}},
+            {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}}};
 }
 
 // Copies the segments' data into a shadow buffer (adjacent segments in one
@@ -419,7 +427,7 @@ bool parse_schedule(const std::string& spec, SortSchedule& out)
         } else {
             return false;
         }
-        if (!(b == 256 || b == 512) || !(it == 4 || it == 8 || it == 16)) return false;
+        if (!(b == 256 || b == 512 || b == 1024) || !(it == 4 || it == 8 || it == 16)) return false;
         c.block = b;
         c.items = it;
         out.push_back(c);
