@@ -172,8 +172,9 @@ def leg_consumers(trtc, native, wl, a, log):
     """The scan / compaction consumers and the comparison sort on 2^28 int32
     items: fused Reduce_By_Key, Inclusive_Scan_By_Key and Unique_Copy over keys in runs of 1..64
     (log-uniform), Partition_Copy / Remove_Copy_If with x % 3 == 0,
-    Transform_Inclusive_Scan (Negate), and Sort with a user comparator (merge
-    path) on 2^24 uint32 keys.  Each GPU result is compared with the oracle's
+    Transform_Inclusive_Scan (Negate), Transform (Negate), Gather (random map),
+    Copy (int32, and int32 -> float32), Min_Element, Merge, and Sort with a
+    user comparator (merge path) on 2^24 uint32 keys.  Each GPU result is compared with the oracle's
     (which is also the timed single-core host baseline)."""
     import numpy as np
     n = 1 << 28
@@ -249,7 +250,7 @@ def leg_consumers(trtc, native, wl, a, log):
     # the other algorithms: element-wise, random gather, index reduction, merge
     def chk_neg(_):
         return (o1.to_host() == -x).all()
-    entry("transform_negate_i32", lambda: trtc.Transform(dx, o1, trtc.Negate()), "trtc_for", lambda _: 8 * n, chk_neg)
+    entry("transform_negate_i32", lambda: trtc.Transform(dx, o1, trtc.Negate()), "trtc_map", lambda _: 8 * n, chk_neg)
     gmap = (wl.u32_uniform(n, 53) % np.uint32(n)).astype(np.int32)
     dmap = trtc.device_vector_from_numpy(gmap)
 
@@ -257,6 +258,16 @@ def leg_consumers(trtc, native, wl, a, log):
         return (o1.to_host() == x[gmap]).all()
     entry("gather_i32_random_map", lambda: trtc.Gather(dmap, dx, o1), "trtc_for", lambda _: 12 * n, chk_gather)
     del dmap
+
+    def chk_copy(_):
+        return (o1.to_host() == x).all()
+    entry("copy_i32", lambda: trtc.Copy(dx, o1), "trtc_map", lambda _: 8 * n, chk_copy)
+    of = trtc.device_vector("float", n)
+
+    def chk_copy_f32(_):
+        return (of.to_host() == x.astype(np.float32)).all()
+    entry("copy_i32_to_f32", lambda: trtc.Copy(dx, of), "trtc_map", lambda _: 8 * n, chk_copy_f32)
+    del of
 
     def chk_min(r):
         return r == int(np.argmin(x))

@@ -100,6 +100,54 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+namespace trtc_host {
+// Element-wise map over [0, n) for bodies that read at index idx and then
+// write at idx: `value` is an expression in `idx` and the named views, `store`
+// a statement that writes `_x` (the value) at idx.  Each thread takes 8
+// indices 256 apart (coalesced), evaluates all 8 values, then stores them, so
+// 8 reads per thread are in flight at once -- a one-index-per-thread For keeps
+// one (2^29 int32 Transform: 0.58 ms as a For).  A thread reads all of its
+// indices before it writes any, and no other thread touches them, so in-place
+// calls (an output that is also an input) are safe.
+bool run_map(const std::vector<const char*>& names, const char* value, const char* store, size_t n, DVList args)
+{
+    if (!ensure_init()) return false;
+    if (n == 0) return true;
+    std::vector<const DeviceViewable*> a(args);
+    std::string sig;
+    for (size_t i = 0; i < names.size(); i++) sig += std::string(", ") + a[i]->name_view_cls() + " " + names[i];
+    std::string src = std::string("extern \"C\" __global__ void __launch_bounds__(256) trtc_map(size_t _trtc_n") + sig +
+                      ")\n{\n"
+                      "    const size_t _trtc_b = (size_t)blockIdx.x * 2048 + threadIdx.x;\n"
+                      "    auto _trtc_val = [&](size_t idx) { return " + value + "; };\n"
+                      "    typedef decltype(_trtc_val((size_t)0)) _trtc_T;\n"
+                      "    auto _trtc_put = [&](size_t idx, const _trtc_T& _x) { " + store + " };\n"
+                      "    if (_trtc_b + 7 * 256 < _trtc_n) {\n"
+                      "        _trtc_T _r[8];\n"
+                      "#pragma unroll\n"
+                      "        for (int _k = 0; _k < 8; _k++) _r[_k] = _trtc_val(_trtc_b + (size_t)_k * 256);\n"
+                      "#pragma unroll\n"
+                      "        for (int _k = 0; _k < 8; _k++) _trtc_put(_trtc_b + (size_t)_k * 256, _r[_k]);\n"
+                      "    } else {\n"
+                      "        for (int _k = 0; _k < 8; _k++) {\n"
+                      "            const size_t idx = _trtc_b + (size_t)_k * 256;\n"
+                      "            if (idx < _trtc_n) _trtc_put(idx, _trtc_val(idx));\n"
+                      "        }\n"
+                      "    }\n"
+                      "}\n";
+    KernelRef k;
+    if (!get_source_kernel(src, "trtc_map", k)) return false;
+    std::vector<ViewBuf> views(names.size());
+    std::vector<void*> ptrs(names.size() + 1);
+    ptrs[0] = &n;
+    for (size_t i = 0; i < names.size(); i++) {
+        views[i] = a[i]->view();
+        ptrs[i + 1] = views[i].data();
+    }
+    return launch_kernel(k, dim3((unsigned)((n + 2047) / 2048)), dim3(256), ptrs.data(), 0, cur_stream());
+}
+}  // namespace trtc_host
+
 // ================================================================ element-wise
 bool TRTC_Fill(DVVectorLike& vec, const DeviceViewable& value)
 {
@@ -109,20 +157,19 @@ bool TRTC_Fill(DVVectorLike& vec, const DeviceViewable& value)
 
 bool TRTC_Transform(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& op)
 {
-    static TRTC_For f({"vin", "vout", "op"}, "idx", "    vout[idx] = (decltype(vout)::value_t)op(vin[idx]);\n");
     if (!need(vec_out, vec_in.size(), "Transform", "vec_out")) return false;
-    return run_for(f, vec_in.size(), {&vec_in, &vec_out, &op});
+    return run_map({"vin", "vout", "op"}, "(decltype(vout)::value_t)op(vin[idx])", "vout[idx] = _x;", vec_in.size(),
+                   {&vec_in, &vec_out, &op});
 }
 
 bool TRTC_Transform_Binary(const DVVectorLike& vec_in1, const DVVectorLike& vec_in2, DVVectorLike& vec_out,
                            const Functor& op)
 {
-    static TRTC_For f({"vin1", "vin2", "vout", "op"}, "idx",
-                      "    vout[idx] = (decltype(vout)::value_t)op(vin1[idx], vin2[idx]);\n");
     if (!need(vec_in2, vec_in1.size(), "Transform_Binary", "vec_in2") ||
         !need(vec_out, vec_in1.size(), "Transform_Binary", "vec_out"))
         return false;
-    return run_for(f, vec_in1.size(), {&vec_in1, &vec_in2, &vec_out, &op});
+    return run_map({"vin1", "vin2", "vout", "op"}, "(decltype(vout)::value_t)op(vin1[idx], vin2[idx])",
+                   "vout[idx] = _x;", vec_in1.size(), {&vec_in1, &vec_in2, &vec_out, &op});
 }
 
 bool TRTC_Transform_If(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& op, const Functor& pred)
@@ -262,6 +309,8 @@ bool TRTC_Tabulate(DVVectorLike& vec, const Functor& op)
 
 bool TRTC_Gather(const DVVectorLike& vec_map, const DVVectorLike& vec_in, DVVectorLike& vec_out)
 {
+    // one index per thread: with 8 random reads in flight per thread (the
+    // element-wise map) a 2^28 random gather ran 5.64 ms against 5.38 ms
     static TRTC_For f({"vmap", "vin", "vout"}, "idx", "    vout[idx] = (decltype(vout)::value_t)vin[vmap[idx]];\n");
     if (!need(vec_out, vec_map.size(), "Gather", "vec_out")) return false;
     return run_for(f, vec_map.size(), {&vec_map, &vec_in, &vec_out});

@@ -166,6 +166,13 @@ bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Func
 
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull);
 
+// Element-wise map over [0, n) for bodies that read at index idx, then write
+// at idx: `value` is an expression in `idx` and the named views (args, same
+// order), `store` a statement that writes `_x` at idx.  8 indices per thread,
+// all read before any is written (in-place calls are safe).
+bool run_map(const std::vector<const char*>& names, const char* value, const char* store, size_t n,
+             std::initializer_list<const DeviceViewable*> args);
+
 }  // namespace trtc_host
 
 #define TRTC_ERR(...) (fprintf(stderr, "[ThrustRTC] " __VA_ARGS__), fprintf(stderr, "\n"))
