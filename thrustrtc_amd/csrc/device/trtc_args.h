@@ -80,6 +80,7 @@ struct MergeArgs
 {
     size_t na;  // length of the first sorted input
     size_t nb;  // length of the second
+    const unsigned long long* splits;  // [tiles + 1] merge-path splits at the tile starts, or null
 };
 
 struct MergeSortArgs

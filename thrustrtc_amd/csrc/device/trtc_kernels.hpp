@@ -204,6 +204,13 @@ __global__ void __launch_bounds__(TRTC_MSORT_BLOCK) trtc_merge_k(AK ak, BK bk, O
     trtc::merge_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, false, K>(ak, bk, ak, bk, ok, ok, comp, a);
 }
 
+template <class AK, class BK, class Comp>
+__global__ void __launch_bounds__(256) trtc_merge_split_k(AK ak, BK bk, Comp comp, trtc::MergeArgs a,
+                                                          unsigned long long* splits)
+{
+    trtc::merge_splits<TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS>(ak, bk, comp, a, splits);
+}
+
 template <class AK, class BK, class AV, class BV, class OK, class OV, class Comp>
 __global__ void __launch_bounds__(TRTC_MSORT_BLOCK)
 trtc_merge_by_key_k(AK ak, BK bk, AV av, BV bv, OK ok, OV ov, Comp comp, trtc::MergeArgs a)

@@ -1,11 +1,14 @@
 // Merge / Merge_By_Key of two sorted inputs (merge.cpp): a stable merge in
 // which equal keys take the first input first (std::merge).
 //
-// One workgroup produces one tile of BLOCK*ITEMS outputs: it finds where the
-// tile's first and last diagonals cut the two inputs (merge-path binary
-// search over the views, ties to the first input), stages the two input
-// ranges in LDS, and every thread merges its ITEMS outputs from its own
-// split, writing keys (and the values of their source elements).
+// One workgroup produces one tile of BLOCK*ITEMS outputs: the merge-path
+// splits of all tile starts come from a split launch (one thread per tile
+// boundary: the binary searches run in parallel instead of two dependent
+// chains of global reads per workgroup; ties go to the first input); the
+// tile stages its two input ranges in LDS, every thread merges its ITEMS
+// outputs from its own split, and the merged keys (and their source
+// positions) go back through LDS so that keys are stored, and values
+// gathered and stored, by consecutive threads at consecutive outputs.
 #pragma once
 #include "trtc_msort.hpp"
 
@@ -25,17 +28,33 @@ TRTC_DEV size_t merge_split_views(AV& A, size_t lenA, BV& B, size_t lenB, size_t
     return lo;
 }
 
+// Splits of the tile starts b*TILE, b = 0 .. tiles (the last one is n).
+template <int TILE, class AK, class BK, class Comp>
+TRTC_DEV void merge_splits(AK& ak, BK& bk, Comp& comp, const MergeArgs& a, unsigned long long* splits)
+{
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = a.na + a.nb;
+    const size_t tiles = (n + TILE - 1) / TILE;
+    if (b > tiles) return;
+    const size_t d = b * TILE < n ? b * TILE : n;
+    splits[b] = merge_split_views(ak, a.na, bk, a.nb, d, comp);
+}
+
 template <int BLOCK, int ITEMS, bool VALS, class K, class AK, class BK, class AV, class BV, class OK, class OV,
           class Comp>
 TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& comp, const MergeArgs& a)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ K s_k[TILE];
+    __shared__ unsigned s_src[TILE];
     __shared__ size_t s_split[2];
     const size_t la = a.na, lb = a.nb, n = la + lb;
     const size_t out0 = (size_t)blockIdx.x * TILE;
     const size_t d1 = out0 + TILE < n ? out0 + TILE : n;
-    if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split_views(ak, la, bk, lb, threadIdx.x ? d1 : out0, comp);
+    if (threadIdx.x < 2) {
+        s_split[threadIdx.x] = a.splits ? (size_t)a.splits[blockIdx.x + threadIdx.x]
+                                        : merge_split_views(ak, la, bk, lb, threadIdx.x ? d1 : out0, comp);
+    }
     __syncthreads();
     const size_t a0 = s_split[0], a1 = s_split[1];
     const size_t b0 = out0 - a0, b1 = d1 - a1;
@@ -44,27 +63,40 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
     __syncthreads();
     const unsigned total = na + nb;
     const unsigned q0 = threadIdx.x * ITEMS;
-    if (q0 >= total) return;
-    unsigned i = (unsigned)merge_split(s_k, na, s_k + na, nb, q0, comp);
-    unsigned j = q0 - i;
-    const unsigned cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
-    typedef typename OK::value_t KO;
+    K k[ITEMS];
+    unsigned src[ITEMS];  // position in the staged ranges: < na from the first input
+    unsigned cnt = 0;
+    if (q0 < total) {
+        unsigned i = (unsigned)merge_split(s_k, na, s_k + na, nb, q0, comp);
+        unsigned j = q0 - i;
+        cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
 #pragma unroll
-    for (int e = 0; e < ITEMS; e++) {
-        if ((unsigned)e >= cnt) break;
-        bool takeA;
-        if (i >= na) takeA = false;
-        else if (j >= nb) takeA = true;
-        else takeA = !comp(s_k[na + j], s_k[i]);
-        const size_t p = out0 + q0 + e;
-        if (takeA) {
-            ok[p] = (KO)s_k[i];
-            if constexpr (VALS) ov[p] = (typename OV::value_t)av[a0 + i];
-            i++;
-        } else {
-            ok[p] = (KO)s_k[na + j];
-            if constexpr (VALS) ov[p] = (typename OV::value_t)bv[b0 + j];
-            j++;
+        for (int e = 0; e < ITEMS; e++) {
+            if ((unsigned)e >= cnt) break;
+            bool takeA;
+            if (i >= na) takeA = false;
+            else if (j >= nb) takeA = true;
+            else takeA = !comp(s_k[na + j], s_k[i]);
+            src[e] = takeA ? i : na + j;
+            k[e] = s_k[src[e]];
+            if (takeA) i++;
+            else j++;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < ITEMS; e++)
+        if ((unsigned)e < cnt) {
+            s_k[q0 + e] = k[e];
+            if constexpr (VALS) s_src[q0 + e] = src[e];
+        }
+    __syncthreads();
+    typedef typename OK::value_t KO;
+    for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
+        ok[out0 + p] = (KO)s_k[p];
+        if constexpr (VALS) {
+            const unsigned q = s_src[p];
+            ov[out0 + p] = q < na ? (typename OV::value_t)av[a0 + q] : (typename OV::value_t)bv[b0 + q - na];
         }
     }
 }

@@ -653,12 +653,27 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
         add(kout);
     }
     add(comp);
-    trtc::MergeArgs a{k1.size(), k2.size()};
+    const size_t tile = 256 * 8;  // TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS
+    const size_t tiles = (n + tile - 1) / tile;
+    hipStream_t s = cur_stream();
+    // merge-path splits of every tile start, all searched in parallel
+    PoolBuf splits((tiles + 1) * 8, s);
+    if (!splits.p) return false;
+    trtc::MergeArgs a{k1.size(), k2.size(), splits.as<unsigned long long>()};
+    {
+        const std::string sname = "trtc_merge_split_k<" + canon_type(k1.name_view_cls()) + ", " +
+                                  canon_type(k2.name_view_cls()) + ", " + canon_type(comp.name_view_cls()) + ">";
+        KernelRef sk;
+        if (!get_template_kernel(sname, sname, sk)) return false;
+        ViewBuf b1 = k1.view(), b2 = k2.view(), bc = comp.view();
+        unsigned long long* sp = splits.as<unsigned long long>();
+        void* sargs[] = {b1.data(), b2.data(), bc.data(), &a, &sp};
+        if (!launch_kernel(sk, dim3((unsigned)((tiles + 1 + 255) / 256)), dim3(256), sargs, 0, s)) return false;
+    }
     args.push_back(&a);
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
-    const size_t tile = 256 * 8;  // TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS
-    return launch_kernel(k, dim3((unsigned)((n + tile - 1) / tile)), dim3(256), args.data(), 0, cur_stream());
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(256), args.data(), 0, s);
 }
 }  // namespace
 
