@@ -27,6 +27,7 @@
 #include "extrema.h"
 #include "fake_vectors/DVCounter.h"
 #include "fake_vectors/DVCustomVector.h"
+#include "fake_vectors/DVZipped.h"
 #include "fill.h"
 #include "find.h"
 #include "for_each.h"
@@ -479,15 +480,22 @@ bool extreme_index(const DVVectorLike& vec, const Functor& comp, bool max, size_
         TRTC_ERR("Min/Max_Element: empty vector");
         return false;
     }
-    // the earlier index wins ties: the first smallest / the first largest
-    Functor op({{"vec", &vec}, {"comp", &comp}}, {"i1", "i2"},
-               max ? "        return comp(vec[i1], vec[i2]) ? i2 : i1;\n"
-                   : "        return comp(vec[i2], vec[i1]) ? i2 : i1;\n");
+    // Reduce (index, value) pairs: the running extreme stays in registers and
+    // every element is read once (reducing the index counter with a functor
+    // that looks both values up read two per combine: 2^28 int32 Min_Element
+    // 1.05 ms).  The ordered reduction always combines (earlier, later), and
+    // the later pair wins only when strictly better, so the earlier index
+    // wins ties: the first smallest / the first largest.  The index is the
+    // first member, at offset 0 of the result.
     uint64_t zero = 0;
     DVUInt64 first(zero);
     DVCounter idx(first, vec.size());
+    DVZipped pairs({&idx, const_cast<DVVectorLike*>(&vec)}, {"index", "value"});
+    Functor op({{"comp", &comp}}, {"a", "b"},
+               max ? "        return comp(a.value, b.value) ? b : a;\n"
+                   : "        return comp(b.value, a.value) ? b : a;\n");
     ViewBuf buf;
-    if (!reduce_view(idx, nullptr, op, buf)) return false;
+    if (!reduce_view(pairs, nullptr, op, buf)) return false;
     uint64_t r;
     memcpy(&r, buf.data(), 8);
     id = (size_t)r;
