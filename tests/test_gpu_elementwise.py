@@ -1,0 +1,114 @@
+"""GPU parity at the edges of the element-wise map kernel (Copy, Transform,
+Transform_Binary: 8 indices per thread in blocks of 2048), of the merge with
+parallel merge-path splits (Merge / Merge_By_Key, 2048-output tiles) and of
+the (index, value) pair reduction behind Min/Max_Element.  Expected values
+come from oracle/algorithms.py (merge: a stable sort of the concatenation,
+ties from the first input first) and numpy element-wise arithmetic; every
+comparison is exact."""
+import numpy as np
+import pytest
+
+from helpers import oracle_algorithms
+
+pytestmark = pytest.mark.gpu
+
+# around the map block (2048 indices) and the merge tile (2048 outputs)
+SIZES = [1, 7, 255, 256, 2047, 2048, 2049, 6143, 100003, (1 << 22) + 7]
+
+
+@pytest.fixture(scope="module")
+def trtc():
+    import thrustrtc_amd
+    return thrustrtc_amd
+
+
+@pytest.fixture(scope="module")
+def oa():
+    return oracle_algorithms()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_copy_and_transform_sizes(trtc, n):
+    say(f"copy / transform n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.integers(-2**31, 2**31 - 1, size=n, dtype=np.int32)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector("int32_t", n)
+    trtc.Copy(d, o)
+    assert np.array_equal(o.to_host(), x)
+    of = trtc.device_vector("float", n)
+    trtc.Copy(d, of)
+    assert np.array_equal(of.to_host(), x.astype(np.float32))
+    trtc.Transform(d, o, trtc.Negate())
+    assert np.array_equal(o.to_host(), (np.zeros_like(x) - x))
+    y = rng.integers(-1000, 1000, size=n, dtype=np.int32)
+    trtc.Transform_Binary(d, trtc.device_vector_from_numpy(y), o, trtc.Plus())
+    assert np.array_equal(o.to_host(), (x + y).astype(np.int32))
+    # in place: the output is the input
+    trtc.Transform(d, d, trtc.Negate())
+    assert np.array_equal(d.to_host(), (np.zeros_like(x) - x))
+
+
+@pytest.mark.parametrize("n", [5, 4097, (1 << 20) + 3])
+def test_copy_widths(trtc, n):
+    say(f"copy widths n={n}")
+    rng = np.random.default_rng(7 + n)
+    b = rng.integers(0, 255, size=n, dtype=np.uint8)  # same-type 1-byte copy (hipMemcpyAsync)
+    o8 = trtc.device_vector("uint8_t", n)
+    trtc.Copy(trtc.device_vector_from_numpy(b), o8)
+    assert np.array_equal(o8.to_host(), b)
+    q = rng.integers(-2**40, 2**40, size=n, dtype=np.int64)
+    od = trtc.device_vector("double", n)
+    trtc.Copy(trtc.device_vector_from_numpy(q), od)
+    assert np.array_equal(od.to_host(), q.astype(np.float64))
+    o64 = trtc.device_vector("int64_t", n)
+    trtc.Copy(trtc.device_vector_from_numpy(q), o64)
+    assert np.array_equal(o64.to_host(), q)
+
+
+# (the reference's Python layer refuses 0-size vectors, so every input holds one item at least)
+@pytest.mark.parametrize("na,nb", [(1, 5), (5, 1), (1, 2048), (2047, 2049), (3000, 17), (100003, 65536),
+                                   ((1 << 21) + 1, (1 << 20) - 3)])
+def test_merge_shapes(trtc, oa, na, nb):
+    say(f"merge na={na} nb={nb}")
+    rng = np.random.default_rng(na * 31 + nb)
+    # few distinct keys: many ties across the two inputs and across tiles
+    a = np.sort(rng.integers(-50, 50, size=na, dtype=np.int32))
+    b = np.sort(rng.integers(-50, 50, size=nb, dtype=np.int32))
+    va = np.arange(na, dtype=np.int32)
+    vb = np.arange(na, na + nb, dtype=np.int32)
+    wk, wv = oa.merge_np(a, b, False, va, vb)
+    da, db = trtc.device_vector_from_numpy(a), trtc.device_vector_from_numpy(b)
+    o = trtc.device_vector("int32_t", na + nb)
+    trtc.Merge(da, db, o)
+    assert np.array_equal(o.to_host(), wk)
+    ko, vo = trtc.device_vector("int32_t", na + nb), trtc.device_vector("int32_t", na + nb)
+    trtc.Merge_By_Key(da, db, trtc.device_vector_from_numpy(va), trtc.device_vector_from_numpy(vb), ko, vo)
+    assert np.array_equal(ko.to_host(), wk) and np.array_equal(vo.to_host(), wv)  # payload pins tie order
+    # descending under Greater
+    ad, bd = a[::-1].copy(), b[::-1].copy()
+    gk, gv = oa.merge_np(ad, bd, True, va, vb)
+    trtc.Merge_By_Key(trtc.device_vector_from_numpy(ad), trtc.device_vector_from_numpy(bd),
+                      trtc.device_vector_from_numpy(va), trtc.device_vector_from_numpy(vb), ko, vo, trtc.Greater())
+    assert np.array_equal(ko.to_host(), gk) and np.array_equal(vo.to_host(), gv)
+
+
+@pytest.mark.parametrize("n", [1, 2, 2049, 100003, (1 << 22) + 5])
+def test_extrema_ties(trtc, n):
+    say(f"extrema ties n={n}")
+    rng = np.random.default_rng(11 + n)
+    x = rng.integers(-3, 4, size=n, dtype=np.int32)  # the extremes occur many times
+    d = trtc.device_vector_from_numpy(x)
+    assert trtc.Min_Element(d) == int(np.argmin(x))  # numpy: the first occurrence
+    assert trtc.Max_Element(d) == int(np.argmax(x))
+    assert tuple(trtc.MinMax_Element(d)) == (int(np.argmin(x)), int(np.argmax(x)))
+    assert trtc.Min_Element(d, trtc.Greater()) == int(np.argmax(x))
+    f = x.astype(np.float64) * 0.5
+    df = trtc.device_vector_from_numpy(f)
+    assert trtc.Min_Element(df) == int(np.argmin(f)) and trtc.Max_Element(df) == int(np.argmax(f))
+    allsame = trtc.device_vector_from_numpy(np.full(n, 9, dtype=np.int32))
+    assert trtc.Min_Element(allsame) == 0 and trtc.Max_Element(allsame) == 0
