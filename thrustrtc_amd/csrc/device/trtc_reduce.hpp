@@ -213,16 +213,19 @@ TRTC_DEV void reduce_streaming(const T* p, Op& op, const T& init, const ReduceAr
     Maybe<T> acc[UNROLL];
 #pragma unroll
     for (int u = 0; u < UNROLL; u++) acc[u].has = 0;
-    for (size_t v0 = tid; v0 < nv; v0 += nthreads * UNROLL) {
+    // each wave reads UNROLL KiB contiguously per step (one 1 KiB piece per
+    // load instruction); the waves take the chunks grid-stride
+    const size_t wave_g = tid / 64, nwaves = nthreads / 64, lane64 = tid & 63;
+    for (size_t c0 = wave_g * (UNROLL * 64) + lane64; c0 < nv; c0 += nwaves * (UNROLL * 64)) {
         v4u x[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
-            size_t v = v0 + (size_t)u * nthreads;
+            size_t v = c0 + (size_t)u * 64;
             if (v < nv) x[u] = __builtin_nontemporal_load(q + v);
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
-            size_t v = v0 + (size_t)u * nthreads;
+            size_t v = c0 + (size_t)u * 64;
             if (v < nv) {
                 T e[V];
                 __builtin_memcpy(e, &x[u], 16);
