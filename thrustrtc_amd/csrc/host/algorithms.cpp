@@ -26,6 +26,7 @@ namespace {
 
 constexpr unsigned kBlock = 256;
 constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: scans, selections, by-key kernels
+constexpr unsigned kCopyIfBlock = 1024;  // TRTC_COPY_IF_BLOCK
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -508,7 +509,9 @@ uint32_t general_copy_if(const DVVectorLike& vin, const DVVectorLike* sten, DVVe
     stencil_arg(args, sten);
     args.view(vout);
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(vin.elem_size()), kScanBlock, 8, sten != nullptr);
+    // items per thread follow the wider of the input and the stencil (kernel: CopyIfItems)
+    const size_t esz = std::max(vin.elem_size(), sten ? sten->elem_size() : (size_t)1);
+    return run_selection(name, args, n, (size_t)kCopyIfBlock * tile_items_for(esz), kCopyIfBlock, 8, sten != nullptr);
 }
 
 // Unique / Unique_By_Key (unique.cpp:5-95): keep item i when i == 0 or
