@@ -207,6 +207,8 @@ def leg_consumers(trtc, native, wl, a, log):
             kms = sum(m_ for _, m_ in found if m_)
         else:
             kname, kms = kernel_ms(prof, prefix)
+        if not kms:
+            raise RuntimeError(f"{name}: no launch of {prefix} recorded during the timed calls")
         algo = algo_fn(out[0])
         r = {"ms": round(dt * 1e3, 4), "kernel": kname, "kernel_ms": round(kms, 4), "algorithmic_bytes": algo,
              "kernel_gbs": round(algo / kms / 1e6, 1), "frac_hbm": round(algo / kms / 1e6 / HBM_PEAK_GBS, 4)}
@@ -276,7 +278,7 @@ def leg_consumers(trtc, native, wl, a, log):
 
     def chk_min(r):
         return r == int(np.argmin(x))
-    entry("min_element_i32", lambda: trtc.Min_Element(dx), "trtc_reduce_k<", lambda _: 4 * n, chk_min)
+    entry("min_element_i32", lambda: trtc.Min_Element(dx), "trtc_extreme_k<", lambda _: 4 * n, chk_min)
     half = n // 2
     sa, sb = np.sort(x[:half]), np.sort(x[half:])
     da, db = trtc.device_vector_from_numpy(sa), trtc.device_vector_from_numpy(sb)

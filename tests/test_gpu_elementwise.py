@@ -112,3 +112,23 @@ def test_extrema_ties(trtc, n):
     assert trtc.Min_Element(df) == int(np.argmin(f)) and trtc.Max_Element(df) == int(np.argmax(f))
     allsame = trtc.device_vector_from_numpy(np.full(n, 9, dtype=np.int32))
     assert trtc.Min_Element(allsame) == 0 and trtc.Max_Element(allsame) == 0
+
+
+@pytest.mark.parametrize("dtype,ctype", [("int8", "int8_t"), ("uint16", "uint16_t"), ("int64", "int64_t"),
+                                         ("uint64", "uint64_t"), ("float32", "float")])
+@pytest.mark.parametrize("n", [3, 17, 4099, (1 << 20) + 3])
+def test_extrema_widths(trtc, dtype, ctype, n):
+    """The streaming extrema kernel reads 16/sizeof(T) elements per load:
+    every width, sizes with a scalar tail, few distinct values (ties)."""
+    say(f"extrema widths {dtype} n={n}")
+    rng = np.random.default_rng(n * 7 + len(dtype))
+    x = rng.integers(0, 5, size=n).astype(dtype)
+    d = trtc.device_vector_from_numpy(x)
+    assert trtc.Min_Element(d) == int(np.argmin(x)) and trtc.Max_Element(d) == int(np.argmax(x))
+    assert trtc.Max_Element(d, trtc.Greater()) == int(np.argmin(x))
+    # the extremes once, at the very end and at the very start
+    y = np.full(n, 2, dtype=dtype)
+    y[-1] = 0
+    y[0] = 4
+    dy = trtc.device_vector_from_numpy(y)
+    assert tuple(trtc.MinMax_Element(dy)) == (n - 1, 0)

@@ -63,6 +63,16 @@ __global__ void __launch_bounds__(TRTC_BLOCK) trtc_reduce_k(Src src, Op op, T in
     trtc::ReduceDispatch<TRTC_BLOCK, Src, Op, T>::template run<trtc::ReduceItems<T>::value>(src, op, init, a);
 }
 
+// Position of the first smallest (MAX = 0) / first largest (MAX = 1) element
+// of a contiguous arithmetic vector; the (index, value) pair lands in a.result.
+template <class Vec, class Comp, int MAX>
+__global__ void __launch_bounds__(TRTC_BLOCK) trtc_extreme_k(Vec vec, Comp comp, trtc::ReduceArgs a)
+{
+    typedef typename Vec::value_t T;
+    static_assert(trtc::IsArith<T>::value, "trtc_extreme_k takes arithmetic vectors");
+    trtc::extreme_streaming<TRTC_BLOCK, T, Comp, MAX != 0>(vec._data, comp, a);
+}
+
 // Copy_If (and Remove) in 1024-thread tiles: 2^29 int32 x % 3 == 0 0.805 ->
 // 0.745 ms against 512 threads x 32 items; items per thread follow the wider
 // of the input and the stencil (the LDS transpose buffer holds either).
@@ -77,13 +87,18 @@ trtc_copy_if_k(In in, Sten sten, Out out, Pred pred, trtc::CopyIfArgs a)
     trtc::copy_if_tile<TRTC_COPY_IF_BLOCK, ITEMS, T>(in, sten, out, pred, a);
 }
 
-// Selections on the compaction machinery (Unique, Partition, Reduce_By_Key).
-// Items per thread follow the wider of the key and value types.
+// Selections on the compaction machinery (Unique, Partition, Reduce_By_Key)
+// and the by-key scan.  Items per thread follow the wider of the key and value
+// (or input and stencil) types.  Unique takes 1024-thread tiles (2^28 int32
+// 0.360 -> 0.339 ms); the others keep 512: at 1024 Reduce_By_Key and
+// Partition measured the same, Scan_By_Key slower (0.70 -> 0.86 ms) and
+// Unique_By_Key spills at 128 VGPRs.
+#define TRTC_SEL_BLOCK 1024
 template <class KIn, class KOut, class Pred>
-__global__ void __launch_bounds__(TRTC_SCAN_BLOCK) trtc_unique_k(KIn kin, KOut kout, Pred pred, trtc::CopyIfArgs a)
+__global__ void __launch_bounds__(TRTC_SEL_BLOCK) trtc_unique_k(KIn kin, KOut kout, Pred pred, trtc::CopyIfArgs a)
 {
     typedef typename KIn::value_t K;
-    trtc::unique_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<K>::value, false, K, K>(kin, kin, kout, kout, pred, a);
+    trtc::unique_tile<TRTC_SEL_BLOCK, trtc::CopyIfItems<K>::value, false, K, K>(kin, kin, kout, kout, pred, a);
 }
 
 template <class KIn, class VIn, class KOut, class VOut, class Pred>
@@ -101,11 +116,11 @@ __global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_partition_k(In in, Sten sten, TOut tout, FOut fout, Pred pred, trtc::CopyIfArgs a)
 {
     typedef typename In::value_t T;
-    trtc::partition_tile<TRTC_SCAN_BLOCK, trtc::CopyIfItems<T>::value, T>(in, sten, tout, fout, pred, a);
+    typedef typename Sten::value_t ST;
+    constexpr int ITEMS = sizeof(T) >= sizeof(ST) ? trtc::CopyIfItems<T>::value : trtc::CopyIfItems<ST>::value;
+    trtc::partition_tile<TRTC_SCAN_BLOCK, ITEMS, T>(in, sten, tout, fout, pred, a);
 }
 
-// The by-key kernels use 512-thread tiles as well (2^28 int32: Reduce_By_Key
-// 1.23 -> 0.95 ms, Scan_By_Key 1.45 -> 1.37 ms against 256-thread tiles).
 template <class Keys, class Vals, class KOut, class VOut, class Pred, class Op, class V>
 __global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_reduce_by_key_k(Keys keys, Vals vals, KOut kout, VOut vout, Pred pred, Op op, trtc::CopyIfArgs a)

@@ -277,6 +277,129 @@ TRTC_DEV void reduce_streaming(const T* p, Op& op, const T& init, const ReduceAr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Min/Max_Element on a contiguous arithmetic vector (extrema.cpp:5-80 ask for
+// the position of the first smallest / first largest element under `comp`).
+// The kernel streams like reduce_streaming: every lane keeps the best
+// (index, value) pair of the elements it reads, in increasing index order per
+// accumulator, so a later element replaces the best one only when it is
+// strictly better.  Pairs from different lanes, waves and workgroups meet in
+// no fixed order, so their combination breaks ties by the lower index: for a
+// comparator that is a strict weak order the result is the first extreme
+// element whatever the grid.
+// ---------------------------------------------------------------------------
+template <class T> struct ExtPair
+{
+    unsigned long long i;  // at offset 0: the host reads the index from there
+    T v;
+};
+
+template <class T, class Comp, bool MAX> struct ExtCombine
+{
+    Comp comp;
+    TRTC_DEV bool better(const T& x, const T& y) { return MAX ? (bool)comp(y, x) : (bool)comp(x, y); }
+    TRTC_DEV ExtPair<T> operator()(const ExtPair<T>& p, const ExtPair<T>& q)
+    {
+        const bool qb = better(q.v, p.v);
+        const bool pb = better(p.v, q.v);
+        return (qb || (!pb && q.i < p.i)) ? q : p;
+    }
+};
+
+template <int BLOCK, class T, class Comp, bool MAX>
+TRTC_DEV void extreme_streaming(const T* p, Comp& comp, const ReduceArgs& a)
+{
+    typedef ExtPair<T> P;
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int V = 16 / sizeof(T);
+    constexpr int UNROLL = 4;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    __shared__ Maybe<P> s_w[WAVES];
+    __shared__ bool s_last;
+    ExtCombine<T, Comp, MAX> op{comp};
+
+    const size_t n = a.n;
+    const size_t tid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    const size_t nthreads = (size_t)gridDim.x * BLOCK;
+    size_t head = ((16 - ((size_t)p & 15)) & 15) / sizeof(T);
+    if (((size_t)p % sizeof(T)) != 0 || head > n) head = n;
+    const size_t nv = (n - head) / V;
+    const size_t tail_begin = head + nv * V;
+    const v4u* q = (const v4u*)(p + head);
+
+    // accumulator u sees increasing indices only: strictly better replaces
+    T bv[UNROLL];
+    unsigned long long bi[UNROLL];
+    bool has[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+        has[u] = false;
+        bv[u] = T();
+        bi[u] = 0;
+    }
+    const size_t wave_g = tid / 64, nwaves = nthreads / 64, lane64 = tid & 63;
+    for (size_t c0 = wave_g * (UNROLL * 64) + lane64; c0 < nv; c0 += nwaves * (UNROLL * 64)) {
+        v4u x[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            size_t v = c0 + (size_t)u * 64;
+            if (v < nv) x[u] = __builtin_nontemporal_load(q + v);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            size_t v = c0 + (size_t)u * 64;
+            if (v < nv) {
+                T e[V];
+                __builtin_memcpy(e, &x[u], 16);
+                const unsigned long long base = head + v * V;
+#pragma unroll
+                for (int k = 0; k < V; k++) {
+                    const bool take = !has[u] || op.better(e[k], bv[u]);
+                    bv[u] = take ? e[k] : bv[u];
+                    bi[u] = take ? base + k : bi[u];
+                    has[u] = true;
+                }
+            }
+        }
+    }
+    Maybe<P> t;
+    t.has = 0;
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++)
+        if (has[u]) fold(t, P{bi[u], bv[u]}, op);
+    for (size_t i = tid; i < head; i += nthreads) fold(t, P{i, p[i]}, op);
+    for (size_t i = tail_begin + tid; i < n; i += nthreads) fold(t, P{i, p[i]}, op);
+    t = wave_tree_reduce(t, op);
+    const unsigned wave = threadIdx.x / 64;
+    if (lane_id() == 0) s_w[wave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Maybe<P> b;
+        b.has = 0;
+        for (int w = 0; w < WAVES; w++)
+            if (s_w[w].has) fold(b, s_w[w].v, op);
+        publish<Maybe<P>>(a.partials, blockIdx.x, b);
+        unsigned d = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (d == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last || wave != 0) return;
+    Maybe<P> tot;
+    tot.has = 0;
+    const unsigned G = gridDim.x;
+    for (unsigned base = 0; base < G; base += 64) {
+        unsigned b = base + lane_id();
+        Maybe<P> pp;
+        pp.has = 0;
+        if (b < G) {
+            while (!try_read<Maybe<P>>(a.partials, b, pp)) __builtin_amdgcn_s_sleep(1);
+        }
+        Maybe<P> c = wave_tree_reduce(pp, op);
+        if (lane_id() == 0 && c.has) fold(tot, c.v, op);
+    }
+    if (lane_id() == 0) *(P*)a.result = tot.v;  // n >= 1: some element was read
+}
+
 template <int BLOCK, class Src, class Op, class T> struct ReduceDispatch
 {
     template <int ITEMS> TRTC_DEV static void run(Src& src, Op& op, const T& init, const ReduceArgs& a)

@@ -27,6 +27,7 @@ namespace {
 constexpr unsigned kBlock = 256;
 constexpr unsigned kScanBlock = 512;  // TRTC_SCAN_BLOCK: scans, selections, by-key kernels
 constexpr unsigned kCopyIfBlock = 1024;  // TRTC_COPY_IF_BLOCK
+constexpr unsigned kSelBlock = 1024;     // TRTC_SEL_BLOCK: Unique without values
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -450,6 +451,50 @@ bool general_reduce(const DVVectorLike& vec, const DeviceViewable* init, const F
     return check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
+// Min/Max_Element on a plain arithmetic vector (extrema.cpp:5-80): one
+// streaming launch that carries (index, value) pairs, index ties going to the
+// lower index (trtc_extreme_k).  `handled` is false for other vectors, which
+// take the ordered reduction of zipped (index, value) pairs instead.
+bool general_extreme(const DVVectorLike& vec, const Functor& comp, bool max, size_t& id, bool& handled)
+{
+    handled = false;
+    const ScalarInfo* si = scalar_info(vec.name_elem_cls());
+    if (!si || si->code > TC_F64) return true;  // bool and other types: not here
+    const std::string view = vcls(vec);
+    if (view.compare(0, 11, "VectorView<") != 0) return true;
+    handled = true;
+    if (!ensure_init()) return false;
+    const size_t n = vec.size();
+    const size_t tsz = si->size;
+    std::string name = "trtc_extreme_k<" + view + ", " + vcls(comp) + ", " + (max ? "1" : "0") + ">";
+    KernelRef k;
+    if (!get_template_kernel(name, name, k)) return false;
+    hipStream_t s = cur_stream();
+    // as the streaming reduce: four 256-thread workgroups per CU at most
+    const size_t chunk = 64 * 16;
+    size_t chunks = (n + chunk - 1) / chunk;
+    size_t blocks = std::min<size_t>((size_t)std::max(1, context().num_cus) * 4, (chunks + 3) / 4);
+    if (blocks == 0) blocks = 1;
+    // partials: Maybe<ExtPair<T>> = {u64 index; T value; unsigned has} <= 24 bytes
+    const size_t W = words_for(align_up(align_up(8 + tsz, 8) + 4, 8));
+    const size_t res_off = 16 + blocks * W * 8;
+    PoolBuf scratch(res_off + 16, s);
+    if (!scratch.p) return false;
+    if (!fill_u32(scratch.p, res_off / 4, 0, s)) return false;
+    trtc::ReduceArgs a{n, 0, (unsigned long long*)((char*)scratch.p + 16), (unsigned*)scratch.p,
+                     (char*)scratch.p + res_off, 0};
+    Args args;
+    args.view(vec);
+    args.view(comp);
+    args.val(a);
+    if (!launch_kernel(k, dim3((unsigned)blocks), dim3(kBlock), args.data(), 0, s)) return false;
+    uint64_t r = 0;
+    if (!check_hip(hipMemcpyAsync(&r, a.result, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync")) return false;
+    if (!check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")) return false;
+    id = (size_t)r;
+    return true;
+}
+
 // ---------------------------------------------------------------- copy_if
 
 // The stencil argument of a selection: the stencil's view, or the one-byte
@@ -543,7 +588,8 @@ uint32_t general_unique(const DVVectorLike& kin, const DVVectorLike* vin, DVVect
         args.view(kout);
     }
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(esz), kScanBlock, 8, false);
+    const unsigned block = vin ? kScanBlock : kSelBlock;
+    return run_selection(name, args, n, (size_t)block * tile_items_for(esz), block, 8, false);
 }
 
 // Stable two-way split (partition.cpp:28-130): items whose predicate holds go
@@ -566,7 +612,9 @@ uint32_t general_partition_copy(const DVVectorLike& vin, const DVVectorLike* ste
     args.view(tout);
     args.view(fout);
     args.view(pred);
-    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(vin.elem_size()), kScanBlock, 8,
+    return run_selection(name, args, n,
+                         (size_t)kScanBlock * tile_items_for(std::max(vin.elem_size(), sten ? sten->elem_size() : (size_t)1)),
+                         kScanBlock, 8,
                          sten != nullptr);
 }
 
@@ -666,6 +714,11 @@ namespace trtc_host {
 bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Functor& op, ViewBuf& ret)
 {
     return general_reduce(vec, init, op, ret);
+}
+
+bool extreme_view(const DVVectorLike& vec, const Functor& comp, bool max, size_t& id, bool& handled)
+{
+    return general_extreme(vec, comp, max, id, handled);
 }
 
 bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,

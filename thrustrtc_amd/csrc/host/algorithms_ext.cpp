@@ -480,7 +480,10 @@ bool extreme_index(const DVVectorLike& vec, const Functor& comp, bool max, size_
         TRTC_ERR("Min/Max_Element: empty vector");
         return false;
     }
-    // Reduce (index, value) pairs: the running extreme stays in registers and
+    bool handled = false;
+    if (!extreme_view(vec, comp, max, id, handled)) return false;
+    if (handled) return true;
+    // Other vectors reduce (index, value) pairs: the running extreme stays in registers and
     // every element is read once (reducing the index counter with a functor
     // that looks both values up read two per combine: 2^28 int32 Min_Element
     // 1.05 ms).  The ordered reduction always combines (earlier, later), and

@@ -163,6 +163,9 @@ void* contiguous_data(const DVVectorLike& v);
 // The single-launch reduction of the primitive core over any view (init
 // first when given; an empty vector without init is an error).
 bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Functor& op, ViewBuf& ret);
+// Min/Max_Element of a plain arithmetic vector in one streaming launch;
+// `handled` is false (and nothing runs) for any other vector.
+bool extreme_view(const DVVectorLike& vec, const Functor& comp, bool max, size_t& id, bool& handled);
 
 uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull);
 
