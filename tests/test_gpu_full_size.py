@@ -121,8 +121,9 @@ def test_reduce_f32_2p29(trtc, wl):
     assert abs(r - g["sum_fp64"]) <= 1e-5 * abs(g["sum_fp64"])
 
 
-# Sizes from 2^25 keys (4-byte pairs) use the 1024-thread digit-pass tiles;
-# the 2^28 cases above cover u32 and f32 keys with u32 values.  These cover the
+# Sizes from 2^22 keys (4-byte pairs; 2^21 with an 8-byte key or value) use
+# the 1024-thread digit-pass tiles; the 2^28 cases above cover u32 and f32
+# keys with u32 values.  These cover the
 # other key / value widths, keys-only and descending order on those tiles, with
 # a partial last tile, against numpy's stable sort (the order std::stable_sort
 # gives: -0.0 and +0.0 compare equal; descending = stable ascending sort of an
@@ -138,7 +139,23 @@ LARGE_N = (1 << 25) + 12345
     (np.float32, np.uint8, True),
 ])
 def test_sort_large_tiles(trtc, kdt, vdt, greater):
-    n = LARGE_N
+    check_sort(trtc, LARGE_N, kdt, vdt, greater)
+
+
+# Either side of the switch to 1024-thread tiles (one tile per CU: 256 CUs x
+# 16384 keys for 4-byte keys and values, x 8192 when either is 8 bytes).
+@pytest.mark.parametrize("n,kdt,vdt,greater", [
+    ((1 << 22) - 1, np.uint32, np.uint32, False),
+    ((1 << 22) + 7, np.uint32, np.uint32, False),
+    ((1 << 22) + 7, np.int32, None, True),
+    ((1 << 21) - 5, np.uint64, np.uint32, False),
+    ((1 << 21) + 5, np.int64, np.float64, True),
+])
+def test_sort_tile_switch(trtc, n, kdt, vdt, greater):
+    check_sort(trtc, n, kdt, vdt, greater)
+
+
+def check_sort(trtc, n, kdt, vdt, greater):
     say(f"large-tile sort {np.dtype(kdt).name} / {None if vdt is None else np.dtype(vdt).name} "
         f"{'Greater' if greater else 'Less'} n={n}")
     rng = np.random.default_rng(7)
