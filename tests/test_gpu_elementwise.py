@@ -71,8 +71,9 @@ def test_copy_widths(trtc, n):
 
 
 # (the reference's Python layer refuses 0-size vectors, so every input holds one item at least)
-@pytest.mark.parametrize("na,nb", [(1, 5), (5, 1), (1, 2048), (2047, 2049), (3000, 17), (100003, 65536),
-                                   ((1 << 21) + 1, (1 << 20) - 3)])
+# tiles: 8192 outputs for Merge (512 x 16), 2048 for Merge_By_Key (256 x 8)
+@pytest.mark.parametrize("na,nb", [(1, 5), (5, 1), (1, 2048), (2047, 2049), (3000, 17), (8191, 1), (4096, 4097),
+                                   (12289, 20000), (100003, 65536), ((1 << 21) + 1, (1 << 20) - 3)])
 def test_merge_shapes(trtc, oa, na, nb):
     say(f"merge na={na} nb={nb}")
     rng = np.random.default_rng(na * 31 + nb)
@@ -132,3 +133,23 @@ def test_extrema_widths(trtc, dtype, ctype, n):
     y[0] = 4
     dy = trtc.device_vector_from_numpy(y)
     assert tuple(trtc.MinMax_Element(dy)) == (n - 1, 0)
+
+
+@pytest.mark.parametrize("na,nb", [(3, 8190), (8193, 8191), (70001, 123457)])
+def test_merge_wide_types(trtc, oa, na, nb):
+    """8-byte keys on the 8192-output Merge tiles (67 KB of LDS) and 8-byte
+    values staged beside the keys in Merge_By_Key."""
+    say(f"merge wide na={na} nb={nb}")
+    rng = np.random.default_rng(na + 3 * nb)
+    a = np.sort(rng.integers(-40, 40, size=na).astype(np.float64) * 0.25)
+    b = np.sort(rng.integers(-40, 40, size=nb).astype(np.float64) * 0.25)
+    va = np.arange(na, dtype=np.int64) * 3
+    vb = np.arange(na, na + nb, dtype=np.int64) * 3
+    wk, wv = oa.merge_np(a, b, False, va, vb)
+    da, db = trtc.device_vector_from_numpy(a), trtc.device_vector_from_numpy(b)
+    o = trtc.device_vector("double", na + nb)
+    trtc.Merge(da, db, o)
+    assert np.array_equal(o.to_host(), wk)
+    vo = trtc.device_vector("int64_t", na + nb)
+    trtc.Merge_By_Key(da, db, trtc.device_vector_from_numpy(va), trtc.device_vector_from_numpy(vb), o, vo)
+    assert np.array_equal(o.to_host(), wk) and np.array_equal(vo.to_host(), wv)

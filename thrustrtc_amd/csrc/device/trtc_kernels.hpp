@@ -217,25 +217,26 @@ __global__ void __launch_bounds__(256) trtc_msort_split_k(const T* kin, Comp com
     trtc::msort_splits<TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS, T>(kin, comp, a);
 }
 
-// Merge / Merge_By_Key of two sorted inputs (views of any kind).
-template <class AK, class BK, class OK, class Comp>
-__global__ void __launch_bounds__(TRTC_MSORT_BLOCK) trtc_merge_k(AK ak, BK bk, OK ok, Comp comp, trtc::MergeArgs a)
+// Merge / Merge_By_Key of two sorted inputs (views of any kind), one tile of
+// BLOCK * ITEMS outputs per workgroup.
+template <class AK, class BK, class OK, class Comp, int BLOCK, int ITEMS>
+__global__ void __launch_bounds__(BLOCK) trtc_merge_k(AK ak, BK bk, OK ok, Comp comp, trtc::MergeArgs a)
 {
     typedef typename AK::value_t K;
-    trtc::merge_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, false, K>(ak, bk, ak, bk, ok, ok, comp, a);
+    trtc::merge_tile<BLOCK, ITEMS, false, K>(ak, bk, ak, bk, ok, ok, comp, a);
 }
 
-template <class AK, class BK, class Comp>
+template <class AK, class BK, class Comp, int TILE>
 __global__ void __launch_bounds__(256) trtc_merge_split_k(AK ak, BK bk, Comp comp, trtc::MergeArgs a,
                                                           unsigned long long* splits)
 {
-    trtc::merge_splits<TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS>(ak, bk, comp, a, splits);
+    trtc::merge_splits<TILE>(ak, bk, comp, a, splits);
 }
 
-template <class AK, class BK, class AV, class BV, class OK, class OV, class Comp>
-__global__ void __launch_bounds__(TRTC_MSORT_BLOCK)
+template <class AK, class BK, class AV, class BV, class OK, class OV, class Comp, int BLOCK, int ITEMS>
+__global__ void __launch_bounds__(BLOCK)
 trtc_merge_by_key_k(AK ak, BK bk, AV av, BV bv, OK ok, OV ov, Comp comp, trtc::MergeArgs a)
 {
     typedef typename AK::value_t K;
-    trtc::merge_tile<TRTC_MSORT_BLOCK, TRTC_MSORT_ITEMS, true, K>(ak, bk, av, bv, ok, ov, comp, a);
+    trtc::merge_tile<BLOCK, ITEMS, true, K>(ak, bk, av, bv, ok, ov, comp, a);
 }

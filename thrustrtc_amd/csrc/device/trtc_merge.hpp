@@ -5,10 +5,10 @@
 // splits of all tile starts come from a split launch (one thread per tile
 // boundary: the binary searches run in parallel instead of two dependent
 // chains of global reads per workgroup; ties go to the first input); the
-// tile stages its two input ranges in LDS, every thread merges its ITEMS
-// outputs from its own split, and the merged keys (and their source
-// positions) go back through LDS so that keys are stored, and values
-// gathered and stored, by consecutive threads at consecutive outputs.
+// tile stages its two input ranges (keys, and values for Merge_By_Key) in
+// LDS, every thread merges its ITEMS outputs from its own split, and the
+// merged keys (and their source positions) go back through LDS so that keys
+// and values are stored by consecutive threads at consecutive outputs.
 #pragma once
 #include "trtc_msort.hpp"
 
@@ -40,13 +40,23 @@ TRTC_DEV void merge_splits(AK& ak, BK& bk, Comp& comp, const MergeArgs& a, unsig
     splits[b] = merge_split_views(ak, a.na, bk, a.nb, d, comp);
 }
 
+// LDS position of staged item p: one pad word per 32 so that a thread's
+// ITEMS consecutive outputs (stride ITEMS between lanes) fall in distinct banks.
+TRTC_DEV unsigned merge_pad(unsigned p) { return p + (p >> 5); }
+
 template <int BLOCK, int ITEMS, bool VALS, class K, class AK, class BK, class AV, class BV, class OK, class OV,
           class Comp>
 TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& comp, const MergeArgs& a)
 {
     constexpr int TILE = BLOCK * ITEMS;
-    __shared__ K s_k[TILE];
-    __shared__ unsigned s_src[TILE];
+    constexpr int PADDED = TILE + TILE / 32;
+    typedef typename OV::value_t VO;
+    // values of up to 8 bytes are staged in LDS beside the keys; wider ones
+    // are gathered from the inputs by source position
+    constexpr bool STAGE_V = VALS && sizeof(VO) <= 8;
+    __shared__ K s_k[PADDED];
+    __shared__ unsigned s_src[VALS ? PADDED : 1];
+    __shared__ VO s_v[STAGE_V ? PADDED : 1];
     __shared__ size_t s_split[2];
     const size_t la = a.na, lb = a.nb, n = la + lb;
     const size_t out0 = (size_t)blockIdx.x * TILE;
@@ -59,15 +69,26 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
     const size_t a0 = s_split[0], a1 = s_split[1];
     const size_t b0 = out0 - a0, b1 = d1 - a1;
     const unsigned na = (unsigned)(a1 - a0), nb = (unsigned)(b1 - b0);
-    for (unsigned p = threadIdx.x; p < na + nb; p += BLOCK) s_k[p] = p < na ? (K)ak[a0 + p] : (K)bk[b0 + p - na];
-    __syncthreads();
     const unsigned total = na + nb;
+    for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
+        s_k[merge_pad(p)] = p < na ? (K)ak[a0 + p] : (K)bk[b0 + p - na];
+        if constexpr (STAGE_V) s_v[merge_pad(p)] = p < na ? (VO)av[a0 + p] : (VO)bv[b0 + p - na];
+    }
+    __syncthreads();
     const unsigned q0 = threadIdx.x * ITEMS;
     K k[ITEMS];
     unsigned src[ITEMS];  // position in the staged ranges: < na from the first input
     unsigned cnt = 0;
     if (q0 < total) {
-        unsigned i = (unsigned)merge_split(s_k, na, s_k + na, nb, q0, comp);
+        // merge-path split of diagonal q0 inside the tile (ties to the first input)
+        unsigned lo = q0 > nb ? q0 - nb : 0;
+        unsigned hi = q0 < na ? q0 : na;
+        while (lo < hi) {
+            const unsigned mid = (lo + hi) >> 1;
+            if (!comp(s_k[merge_pad(na + q0 - mid - 1)], s_k[merge_pad(mid)])) lo = mid + 1;
+            else hi = mid;
+        }
+        unsigned i = lo;
         unsigned j = q0 - i;
         cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
 #pragma unroll
@@ -76,9 +97,9 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
             bool takeA;
             if (i >= na) takeA = false;
             else if (j >= nb) takeA = true;
-            else takeA = !comp(s_k[na + j], s_k[i]);
+            else takeA = !comp(s_k[merge_pad(na + j)], s_k[merge_pad(i)]);
             src[e] = takeA ? i : na + j;
-            k[e] = s_k[src[e]];
+            k[e] = s_k[merge_pad(src[e])];
             if (takeA) i++;
             else j++;
         }
@@ -87,16 +108,17 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
 #pragma unroll
     for (int e = 0; e < ITEMS; e++)
         if ((unsigned)e < cnt) {
-            s_k[q0 + e] = k[e];
-            if constexpr (VALS) s_src[q0 + e] = src[e];
+            s_k[merge_pad(q0 + e)] = k[e];
+            if constexpr (VALS) s_src[merge_pad(q0 + e)] = src[e];
         }
     __syncthreads();
     typedef typename OK::value_t KO;
     for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
-        ok[out0 + p] = (KO)s_k[p];
+        ok[out0 + p] = (KO)s_k[merge_pad(p)];
         if constexpr (VALS) {
-            const unsigned q = s_src[p];
-            ov[out0 + p] = q < na ? (typename OV::value_t)av[a0 + q] : (typename OV::value_t)bv[b0 + q - na];
+            const unsigned q = s_src[merge_pad(p)];
+            if constexpr (STAGE_V) ov[out0 + p] = s_v[merge_pad(q)];
+            else ov[out0 + p] = q < na ? (VO)av[a0 + q] : (VO)bv[b0 + q - na];
         }
     }
 }

@@ -638,6 +638,13 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
     }
     if (vout && (!need(*v1, k1.size(), "Merge_By_Key", "value1") || !need(*v2, k2.size(), "Merge_By_Key", "value2")))
         return false;
+    // tile of trtc_merge_k / trtc_merge_by_key_k (2 x 2^27 int32, per call):
+    // Merge 512 x 16 0.63 ms (256 x 8 0.70); Merge_By_Key 256 x 8 1.02 ms
+    // (16 items: 1.5-2.0 ms, three LDS buffers halve the workgroups per CU).
+    // Keys wider than 8 bytes keep 2048-output tiles (LDS).
+    const bool big = vout != nullptr || k1.elem_size() > 8;
+    const unsigned mblock = big ? 256 : 512, mitems = big ? 8 : 16;
+    const std::string shape = ", " + std::to_string(mblock) + ", " + std::to_string(mitems) + ">";
     std::string name;
     std::deque<ViewBuf> bufs;
     std::vector<void*> args;
@@ -649,7 +656,7 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
         name = "trtc_merge_by_key_k<" + canon_type(k1.name_view_cls()) + ", " + canon_type(k2.name_view_cls()) + ", " +
                canon_type(v1->name_view_cls()) + ", " + canon_type(v2->name_view_cls()) + ", " +
                canon_type(kout.name_view_cls()) + ", " + canon_type(vout->name_view_cls()) + ", " +
-               canon_type(comp.name_view_cls()) + ">";
+               canon_type(comp.name_view_cls()) + shape;
         add(k1);
         add(k2);
         add(*v1);
@@ -658,13 +665,13 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
         add(*vout);
     } else {
         name = "trtc_merge_k<" + canon_type(k1.name_view_cls()) + ", " + canon_type(k2.name_view_cls()) + ", " +
-               canon_type(kout.name_view_cls()) + ", " + canon_type(comp.name_view_cls()) + ">";
+               canon_type(kout.name_view_cls()) + ", " + canon_type(comp.name_view_cls()) + shape;
         add(k1);
         add(k2);
         add(kout);
     }
     add(comp);
-    const size_t tile = 256 * 8;  // TRTC_MSORT_BLOCK * TRTC_MSORT_ITEMS
+    const size_t tile = (size_t)mblock * mitems;
     const size_t tiles = (n + tile - 1) / tile;
     hipStream_t s = cur_stream();
     // merge-path splits of every tile start, all searched in parallel
@@ -673,7 +680,8 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
     trtc::MergeArgs a{k1.size(), k2.size(), splits.as<unsigned long long>()};
     {
         const std::string sname = "trtc_merge_split_k<" + canon_type(k1.name_view_cls()) + ", " +
-                                  canon_type(k2.name_view_cls()) + ", " + canon_type(comp.name_view_cls()) + ">";
+                                  canon_type(k2.name_view_cls()) + ", " + canon_type(comp.name_view_cls()) + ", " +
+                                  std::to_string(tile) + ">";
         KernelRef sk;
         if (!get_template_kernel(sname, sname, sk)) return false;
         ViewBuf b1 = k1.view(), b2 = k2.view(), bc = comp.view();
@@ -684,7 +692,7 @@ bool merge_views(const DVVectorLike& k1, const DVVectorLike& k2, const DVVectorL
     args.push_back(&a);
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
-    return launch_kernel(k, dim3((unsigned)tiles), dim3(256), args.data(), 0, s);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(mblock), args.data(), 0, s);
 }
 }  // namespace
 
