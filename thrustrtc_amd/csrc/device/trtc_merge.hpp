@@ -40,16 +40,12 @@ TRTC_DEV void merge_splits(AK& ak, BK& bk, Comp& comp, const MergeArgs& a, unsig
     splits[b] = merge_split_views(ak, a.na, bk, a.nb, d, comp);
 }
 
-// LDS position of staged item p: one pad word per 32 so that a thread's
-// ITEMS consecutive outputs (stride ITEMS between lanes) fall in distinct banks.
-TRTC_DEV unsigned merge_pad(unsigned p) { return p + (p >> 5); }
-
 template <int BLOCK, int ITEMS, bool VALS, class K, class AK, class BK, class AV, class BV, class OK, class OV,
           class Comp>
 TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& comp, const MergeArgs& a)
 {
     constexpr int TILE = BLOCK * ITEMS;
-    constexpr int PADDED = TILE + TILE / 32;
+    constexpr int PADDED = LdsPadded<TILE>::n;
     typedef typename OV::value_t VO;
     // values of up to 8 bytes are staged in LDS beside the keys; wider ones
     // are gathered from the inputs by source position
@@ -71,8 +67,8 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
     const unsigned na = (unsigned)(a1 - a0), nb = (unsigned)(b1 - b0);
     const unsigned total = na + nb;
     for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
-        s_k[merge_pad(p)] = p < na ? (K)ak[a0 + p] : (K)bk[b0 + p - na];
-        if constexpr (STAGE_V) s_v[merge_pad(p)] = p < na ? (VO)av[a0 + p] : (VO)bv[b0 + p - na];
+        s_k[lds_pad(p)] = p < na ? (K)ak[a0 + p] : (K)bk[b0 + p - na];
+        if constexpr (STAGE_V) s_v[lds_pad(p)] = p < na ? (VO)av[a0 + p] : (VO)bv[b0 + p - na];
     }
     __syncthreads();
     const unsigned q0 = threadIdx.x * ITEMS;
@@ -80,15 +76,7 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
     unsigned src[ITEMS];  // position in the staged ranges: < na from the first input
     unsigned cnt = 0;
     if (q0 < total) {
-        // merge-path split of diagonal q0 inside the tile (ties to the first input)
-        unsigned lo = q0 > nb ? q0 - nb : 0;
-        unsigned hi = q0 < na ? q0 : na;
-        while (lo < hi) {
-            const unsigned mid = (lo + hi) >> 1;
-            if (!comp(s_k[merge_pad(na + q0 - mid - 1)], s_k[merge_pad(mid)])) lo = mid + 1;
-            else hi = mid;
-        }
-        unsigned i = lo;
+        unsigned i = lds_split(s_k, 0, na, na, nb, q0, comp);  // ties to the first input
         unsigned j = q0 - i;
         cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
 #pragma unroll
@@ -97,9 +85,9 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
             bool takeA;
             if (i >= na) takeA = false;
             else if (j >= nb) takeA = true;
-            else takeA = !comp(s_k[merge_pad(na + j)], s_k[merge_pad(i)]);
+            else takeA = !comp(s_k[lds_pad(na + j)], s_k[lds_pad(i)]);
             src[e] = takeA ? i : na + j;
-            k[e] = s_k[merge_pad(src[e])];
+            k[e] = s_k[lds_pad(src[e])];
             if (takeA) i++;
             else j++;
         }
@@ -108,16 +96,16 @@ TRTC_DEV void merge_tile(AK& ak, BK& bk, AV& av, BV& bv, OK& ok, OV& ov, Comp& c
 #pragma unroll
     for (int e = 0; e < ITEMS; e++)
         if ((unsigned)e < cnt) {
-            s_k[merge_pad(q0 + e)] = k[e];
-            if constexpr (VALS) s_src[merge_pad(q0 + e)] = src[e];
+            s_k[lds_pad(q0 + e)] = k[e];
+            if constexpr (VALS) s_src[lds_pad(q0 + e)] = src[e];
         }
     __syncthreads();
     typedef typename OK::value_t KO;
     for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
-        ok[out0 + p] = (KO)s_k[merge_pad(p)];
+        ok[out0 + p] = (KO)s_k[lds_pad(p)];
         if constexpr (VALS) {
-            const unsigned q = s_src[merge_pad(p)];
-            if constexpr (STAGE_V) ov[out0 + p] = s_v[merge_pad(q)];
+            const unsigned q = s_src[lds_pad(p)];
+            if constexpr (STAGE_V) ov[out0 + p] = s_v[lds_pad(q)];
             else ov[out0 + p] = q < na ? (VO)av[a0 + q] : (VO)bv[b0 + q - na];
         }
     }

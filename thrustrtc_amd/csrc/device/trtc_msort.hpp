@@ -31,18 +31,41 @@ TRTC_DEV size_t merge_split(const T* A, size_t lenA, const T* B, size_t lenB, si
     return lo;
 }
 
+// LDS position of staged item p: one pad word per 32, so that the ITEMS
+// consecutive items of a thread (stride ITEMS between lanes) fall in distinct
+// banks when every thread reads or writes its e-th item at once.
+TRTC_DEV unsigned lds_pad(unsigned p) { return p + (p >> 5); }
+template <int TILE> struct LdsPadded
+{
+    static constexpr int n = TILE + TILE / 32;
+};
+
+// merge_split over padded LDS: A = s[pad(a + i)], i < la; B = s[pad(b + j)], j < lb.
+template <class T, class Comp>
+TRTC_DEV unsigned lds_split(const T* s, unsigned a, unsigned la, unsigned b, unsigned lb, unsigned d, Comp& comp)
+{
+    unsigned lo = d > lb ? d - lb : 0;
+    unsigned hi = d < la ? d : la;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (!comp(s[lds_pad(b + d - mid - 1)], s[lds_pad(a + mid)])) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <int BLOCK, int ITEMS, class T, class Comp>
 TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs& a)
 {
     constexpr int TILE = BLOCK * ITEMS;
-    __shared__ T s_k[TILE];
-    __shared__ unsigned s_i[TILE];
+    __shared__ T s_k[LdsPadded<TILE>::n];
+    __shared__ unsigned s_i[LdsPadded<TILE>::n];
     const size_t base = (size_t)blockIdx.x * TILE;
     const unsigned valid = (unsigned)((a.n - base) < (size_t)TILE ? (a.n - base) : (size_t)TILE);
 
     for (unsigned p = threadIdx.x; p < valid; p += BLOCK) {
-        s_k[p] = keys[base + p];
-        s_i[p] = a.with_idx ? idx[base + p] : 0u;
+        s_k[lds_pad(p)] = keys[base + p];
+        s_i[lds_pad(p)] = a.with_idx ? idx[base + p] : 0u;
     }
     __syncthreads();
     // 1. each thread: stable sort of its ITEMS consecutive keys (odd-even
@@ -55,8 +78,8 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
 #pragma unroll
     for (int e = 0; e < ITEMS; e++)
         if ((unsigned)e < mine) {
-            k[e] = s_k[p0 + e];
-            ix[e] = s_i[p0 + e];
+            k[e] = s_k[lds_pad(p0 + e)];
+            ix[e] = s_i[lds_pad(p0 + e)];
         }
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
@@ -76,8 +99,8 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
 #pragma unroll
     for (int e = 0; e < ITEMS; e++)
         if ((unsigned)e < mine) {
-            s_k[p0 + e] = k[e];
-            s_i[p0 + e] = ix[e];
+            s_k[lds_pad(p0 + e)] = k[e];
+            s_i[lds_pad(p0 + e)] = ix[e];
         }
     __syncthreads();
     // 2. pairwise run merges inside the tile, by merge path: the thread that
@@ -92,9 +115,7 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
             const unsigned lb = g + run < valid ? ((g + 2 * run <= valid ? 2 * run : valid - g) - run) : 0;
             const unsigned d = p0 - g;
             cnt = la + lb - d < (unsigned)ITEMS ? la + lb - d : (unsigned)ITEMS;
-            const T* A = s_k + g;
-            const T* B = s_k + g + la;
-            unsigned i = (unsigned)merge_split(A, la, B, lb, d, comp);
+            unsigned i = lds_split(s_k, g, la, g + la, lb, d, comp);
             unsigned j = d - i;
 #pragma unroll
             for (int e = 0; e < ITEMS; e++) {
@@ -102,30 +123,26 @@ TRTC_DEV void msort_tile(T* keys, unsigned* idx, Comp& comp, const MergeSortArgs
                 bool takeA;
                 if (i >= la) takeA = false;
                 else if (j >= lb) takeA = true;
-                else takeA = !comp(B[j], A[i]);
-                if (takeA) {
-                    k[e] = A[i];
-                    ix[e] = s_i[g + i];
-                    i++;
-                } else {
-                    k[e] = B[j];
-                    ix[e] = s_i[g + la + j];
-                    j++;
-                }
+                else takeA = !comp(s_k[lds_pad(g + la + j)], s_k[lds_pad(g + i)]);
+                const unsigned q = lds_pad(takeA ? g + i : g + la + j);
+                k[e] = s_k[q];
+                ix[e] = s_i[q];
+                if (takeA) i++;
+                else j++;
             }
         }
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < ITEMS; e++)
             if ((unsigned)e < cnt) {
-                s_k[p0 + e] = k[e];
-                s_i[p0 + e] = ix[e];
+                s_k[lds_pad(p0 + e)] = k[e];
+                s_i[lds_pad(p0 + e)] = ix[e];
             }
         __syncthreads();
     }
     for (unsigned p = threadIdx.x; p < valid; p += BLOCK) {
-        keys[base + p] = s_k[p];
-        if (a.with_idx) idx[base + p] = s_i[p];
+        keys[base + p] = s_k[lds_pad(p)];
+        if (a.with_idx) idx[base + p] = s_i[lds_pad(p)];
     }
 }
 
@@ -156,8 +173,8 @@ TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* 
                           const MergeSortArgs& a)
 {
     constexpr int TILE = BLOCK * ITEMS;
-    __shared__ T s_k[TILE];
-    __shared__ unsigned s_i[TILE];
+    __shared__ T s_k[LdsPadded<TILE>::n];
+    __shared__ unsigned s_i[LdsPadded<TILE>::n];
     __shared__ size_t s_split[2];
     const size_t out0 = (size_t)blockIdx.x * TILE;
     size_t pair, lenA, lenB;
@@ -182,11 +199,11 @@ TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* 
     const unsigned na = (unsigned)(a1 - a0), nb = (unsigned)(b1 - b0);
     for (unsigned p = threadIdx.x; p < na + nb; p += BLOCK) {
         if (p < na) {
-            s_k[p] = A[a0 + p];
-            s_i[p] = a.with_idx ? iin[pair + a0 + p] : 0u;
+            s_k[lds_pad(p)] = A[a0 + p];
+            s_i[lds_pad(p)] = a.with_idx ? iin[pair + a0 + p] : 0u;
         } else {
-            s_k[p] = B[b0 + p - na];
-            s_i[p] = a.with_idx ? iin[pair + lenA + b0 + p - na] : 0u;
+            s_k[lds_pad(p)] = B[b0 + p - na];
+            s_i[lds_pad(p)] = a.with_idx ? iin[pair + lenA + b0 + p - na] : 0u;
         }
     }
     __syncthreads();
@@ -197,7 +214,7 @@ TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* 
     unsigned ix[ITEMS];
     unsigned cnt = 0;
     if (q0 < total) {
-        unsigned i = (unsigned)merge_split(s_k, na, s_k + na, nb, q0, comp);
+        unsigned i = lds_split(s_k, 0, na, na, nb, q0, comp);
         unsigned j = q0 - i;
         cnt = total - q0 < (unsigned)ITEMS ? total - q0 : (unsigned)ITEMS;
 #pragma unroll
@@ -206,29 +223,25 @@ TRTC_DEV void msort_merge(const T* kin, const unsigned* iin, T* kout, unsigned* 
             bool takeA;
             if (i >= na) takeA = false;
             else if (j >= nb) takeA = true;
-            else takeA = !comp(s_k[na + j], s_k[i]);
-            if (takeA) {
-                k[e] = s_k[i];
-                ix[e] = s_i[i];
-                i++;
-            } else {
-                k[e] = s_k[na + j];
-                ix[e] = s_i[na + j];
-                j++;
-            }
+            else takeA = !comp(s_k[lds_pad(na + j)], s_k[lds_pad(i)]);
+            const unsigned q = lds_pad(takeA ? i : na + j);
+            k[e] = s_k[q];
+            ix[e] = s_i[q];
+            if (takeA) i++;
+            else j++;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < ITEMS; e++)
         if ((unsigned)e < cnt) {
-            s_k[q0 + e] = k[e];
-            s_i[q0 + e] = ix[e];
+            s_k[lds_pad(q0 + e)] = k[e];
+            s_i[lds_pad(q0 + e)] = ix[e];
         }
     __syncthreads();
     for (unsigned p = threadIdx.x; p < total; p += BLOCK) {
-        kout[out0 + p] = s_k[p];
-        if (a.with_idx) iout[out0 + p] = s_i[p];
+        kout[out0 + p] = s_k[lds_pad(p)];
+        if (a.with_idx) iout[out0 + p] = s_i[lds_pad(p)];
     }
 }
 
