@@ -153,3 +153,25 @@ def test_merge_wide_types(trtc, oa, na, nb):
     vo = trtc.device_vector("int64_t", na + nb)
     trtc.Merge_By_Key(da, db, trtc.device_vector_from_numpy(va), trtc.device_vector_from_numpy(vb), o, vo)
     assert np.array_equal(o.to_host(), wk) and np.array_equal(vo.to_host(), wv)
+
+
+@pytest.mark.parametrize("n", [1, 16383, 16384, 16385, 3 * 16384 + 1, (1 << 22) + 3])
+def test_scan_8byte_tiles(trtc, n):
+    """Whole-input scans of 8-byte elements run 1024 x 16 = 16384-element
+    tiles: tile edges, inclusive and exclusive, int64 and float64 holding
+    integers (every partial sum exact, so the comparison is exact)."""
+    say(f"8-byte scan n={n}")
+    rng = np.random.default_rng(n + 99)
+    x = rng.integers(-2**40, 2**40, size=n, dtype=np.int64)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector("int64_t", n)
+    trtc.Inclusive_Scan(d, o)
+    assert np.array_equal(o.to_host(), np.cumsum(x))
+    trtc.Exclusive_Scan(d, o)
+    ex = np.concatenate([[0], np.cumsum(x)[:-1]]).astype(np.int64)
+    assert np.array_equal(o.to_host(), ex)
+    f = rng.integers(-1000, 1000, size=n).astype(np.float64)
+    df = trtc.device_vector_from_numpy(f)
+    of = trtc.device_vector("double", n)
+    trtc.Inclusive_Scan(df, of)
+    assert np.array_equal(of.to_host(), np.cumsum(f))

@@ -51,10 +51,18 @@ extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_iota_u32(unsigned*
 // bytes): half as many lookbacks as 256-thread tiles.
 #define TRTC_SCAN_BLOCK 512
 
-template <class Src, class Out, class Op, class T>
-__global__ void __launch_bounds__(TRTC_SCAN_BLOCK) trtc_scan_k(Src src, Out out, Op op, T init, trtc::ScanArgs a)
+// The whole-input scan takes 1024-thread tiles for 8-byte elements (2^28
+// int64: 1.05 -> 0.90 ms against 512 x 16); 4-byte ones keep 512 x 32 (1024
+// x 32: 0.88 -> 0.89-0.93 ms), wider ones 512 threads for LDS.
+template <class T> struct ScanBlock
 {
-    trtc::scan_tile<TRTC_SCAN_BLOCK, trtc::ScanItems<T>::value, T>(src, out, op, init, a);
+    static constexpr int value = sizeof(T) == 8 ? 1024 : TRTC_SCAN_BLOCK;
+};
+
+template <class Src, class Out, class Op, class T>
+__global__ void __launch_bounds__(ScanBlock<T>::value) trtc_scan_k(Src src, Out out, Op op, T init, trtc::ScanArgs a)
+{
+    trtc::scan_tile<ScanBlock<T>::value, trtc::ScanItems<T>::value, T>(src, out, op, init, a);
 }
 
 template <class Src, class Op, class T>

@@ -389,7 +389,8 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kScanBlock * tile_items_for(tsz);
+    const unsigned block = tsz == 8 ? 1024u : kScanBlock;  // ScanBlock<T> in the device header
+    size_t tile = (size_t)block * tile_items_for(tsz);
     size_t tiles = (n + tile - 1) / tile;
     LookbackScratch lb;
     if (!lb.make(tiles, tsz, s)) return false;
@@ -400,7 +401,7 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     args.view(op);
     args.raw(initv.data(), initv.size());
     args.val(a);
-    return launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(block), args.data(), 0, s);
 }
 
 // ---------------------------------------------------------------- reduce
