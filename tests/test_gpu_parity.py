@@ -360,6 +360,26 @@ def test_scan_by_key_parity(trtc, orc, wl, n):
     assert (dv.to_host() == orc.scan_by_key(keys, vals, "int32_t", "int32_t", True, 0)).all()
 
 
+# 8-byte keys or values run 1024 x 16 = 16384-item tiles: edges of those tiles
+@pytest.mark.parametrize("n", [16383, 16384, 16385, 3 * 16384 + 7, 200003])
+@pytest.mark.parametrize("kelem,velem", [("int64_t", "int64_t"), ("int32_t", "double"), ("int64_t", "int32_t")])
+def test_scan_by_key_8byte_tiles(trtc, orc, wl, n, kelem, velem):
+    say("scan_by_key 8-byte", kelem, velem, n)
+    sizes, _ = wl.segment_sizes(n, 13, 1, 3000)
+    keys = np.repeat(np.arange(len(sizes)), sizes.astype(np.int64))[:n].astype(NP_OF[kelem])
+    vals = wl.i32_range(n, 14, -1000, 1000).astype(NP_OF[velem])  # integers: exact in double too
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(vals)
+    o = trtc.device_vector(velem, n)
+    trtc.Inclusive_Scan_By_Key(dk, dv, o)
+    assert (o.to_host() == orc.scan_by_key(keys, vals, kelem, velem)).all()
+    trtc.Exclusive_Scan_By_Key(dk, dv, o)
+    assert (o.to_host() == orc.scan_by_key(keys, vals, kelem, velem, True, 0)).all()
+
+
+NP_OF = {"int32_t": np.int32, "int64_t": np.int64, "double": np.float64}
+
+
 def test_scan_by_key_float_keys_and_max(trtc, orc):
     keys = np.array([0.0, -0.0, 1.0, 1.0, 2.0, -0.0] * 3000, dtype=np.float32)
     vals = np.arange(len(keys), dtype=np.int32) % 13

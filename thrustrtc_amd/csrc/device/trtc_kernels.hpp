@@ -109,6 +109,15 @@ __global__ void __launch_bounds__(TRTC_SEL_BLOCK) trtc_unique_k(KIn kin, KOut ko
     trtc::unique_tile<TRTC_SEL_BLOCK, trtc::CopyIfItems<K>::value, false, K, K>(kin, kin, kout, kout, pred, a);
 }
 
+// Block of the by-key scan: 1024 threads when the wider of the key and value
+// types is 8 bytes (2^27 int64 keys and values 0.81 -> 0.76 ms), as the
+// whole-input scan.  Unique_By_Key and Partition measured slower that way
+// (0.62 -> 0.64, 0.68 -> 0.71 ms) and Reduce_By_Key the same; they keep 512.
+template <class A, class B> struct WideBlock
+{
+    static constexpr int value = (sizeof(A) >= sizeof(B) ? sizeof(A) : sizeof(B)) == 8 ? 1024 : TRTC_SCAN_BLOCK;
+};
+
 template <class KIn, class VIn, class KOut, class VOut, class Pred>
 __global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
 trtc_unique_by_key_k(KIn kin, VIn vin, KOut kout, VOut vout, Pred pred, trtc::CopyIfArgs a)
@@ -139,12 +148,12 @@ trtc_reduce_by_key_k(Keys keys, Vals vals, KOut kout, VOut vout, Pred pred, Op o
 }
 
 template <class Keys, class Vals, class Out, class Pred, class Op, class V>
-__global__ void __launch_bounds__(TRTC_SCAN_BLOCK)
+__global__ void __launch_bounds__((WideBlock<typename Keys::value_t, V>::value))
 trtc_scan_by_key_k(Keys keys, Vals vals, Out out, Pred pred, Op op, V init, trtc::ScanByKeyArgs a)
 {
     typedef typename Keys::value_t K;
     constexpr int ITEMS = sizeof(K) >= sizeof(V) ? trtc::ScanByKeyItems<K>::value : trtc::ScanByKeyItems<V>::value;
-    trtc::scan_by_key_tile<TRTC_SCAN_BLOCK, ITEMS, V, K>(keys, vals, out, pred, op, init, a);
+    trtc::scan_by_key_tile<WideBlock<K, V>::value, ITEMS, V, K>(keys, vals, out, pred, op, init, a);
 }
 
 template <int KB>

@@ -37,6 +37,9 @@ int items_for(size_t elem_size) { return elem_size <= 4 ? 16 : (elem_size <= 8 ?
 // by-key scan and reduction: 128 bytes per lane (must match ScanItems,
 // CopyIfItems and ScanByKeyItems in the device headers).
 int tile_items_for(size_t elem_size) { return elem_size <= 4 ? 32 : (elem_size <= 8 ? 16 : 4); }
+// Block of Scan_By_Key from the wider of the key and value types (WideBlock
+// in the device header).
+unsigned wide_block(size_t elem_size) { return elem_size == 8 ? 1024u : kScanBlock; }
 size_t words_for(size_t bytes) { return (bytes + 3) / 4; }
 
 std::string vcls(const DeviceViewable& v) { return canon_type(v.name_view_cls()); }
@@ -615,10 +618,8 @@ uint32_t general_partition_copy(const DVVectorLike& vin, const DVVectorLike* ste
     args.view(tout);
     args.view(fout);
     args.view(pred);
-    return run_selection(name, args, n,
-                         (size_t)kScanBlock * tile_items_for(std::max(vin.elem_size(), sten ? sten->elem_size() : (size_t)1)),
-                         kScanBlock, 8,
-                         sten != nullptr);
+    const size_t esz = std::max(vin.elem_size(), sten ? sten->elem_size() : (size_t)1);
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(esz), kScanBlock, 8, sten != nullptr);
 }
 
 // In-place partition: selected items move forward in place (each lands at or
@@ -669,8 +670,9 @@ uint32_t general_reduce_by_key(const DVVectorLike& kin, const DVVectorLike& vin,
     args.view(pred);
     args.view(op);
     // state {V v; unsigned flag; unsigned tails;} fits align_up(V, 8) + 8 bytes
-    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(std::max(kin.elem_size(), vsz)), kScanBlock,
-                         align_up(vsz, 8) + 8, false);
+    const size_t esz = std::max(kin.elem_size(), vsz);
+    return run_selection(name, args, n, (size_t)kScanBlock * tile_items_for(esz), kScanBlock, align_up(vsz, 8) + 8,
+                         false);
 }
 
 // ---------------------------------------------------------------- scan_by_key
@@ -694,7 +696,8 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
     KernelRef k;
     if (!get_template_kernel(name, name, k)) return false;
     hipStream_t s = cur_stream();
-    size_t tile = (size_t)kScanBlock * tile_items_for(std::max(keys.elem_size(), vsz));
+    const unsigned block = wide_block(std::max(keys.elem_size(), vsz));
+    size_t tile = (size_t)block * tile_items_for(std::max(keys.elem_size(), vsz));
     size_t tiles = (n + tile - 1) / tile;
     size_t seg_bytes = align_up(vsz + 4, std::min<size_t>(std::max<size_t>(vsz, 4), 8));
     LookbackScratch lb;
@@ -708,7 +711,7 @@ bool general_scan_by_key(const DVVectorLike& keys, const DVVectorLike& vals, DVV
     args.view(op);
     args.raw(initv.data(), initv.size());
     args.val(a);
-    return launch_kernel(k, dim3((unsigned)tiles), dim3(kScanBlock), args.data(), 0, s);
+    return launch_kernel(k, dim3((unsigned)tiles), dim3(block), args.data(), 0, s);
 }
 
 }  // namespace
