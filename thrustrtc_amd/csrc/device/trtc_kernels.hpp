@@ -156,10 +156,13 @@ trtc_scan_by_key_k(Keys keys, Vals vals, Out out, Pred pred, Op op, V init, trtc
     trtc::scan_by_key_tile<WideBlock<K, V>::value, ITEMS, V, K>(keys, vals, out, pred, op, init, a);
 }
 
+// Histogram pre-pass: 1024 threads, one workgroup per CU (up to 128 KB of
+// LDS counters).
+#define TRTC_HIST_BLOCK 1024
 template <int KB>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_hist_k(trtc::RadixHistArgs a)
+__global__ void __launch_bounds__(TRTC_HIST_BLOCK) trtc_radix_hist_k(trtc::RadixHistArgs a)
 {
-    trtc::radix_hist<TRTC_BLOCK, KB>(a);
+    trtc::radix_hist<TRTC_HIST_BLOCK, KB>(a);
 }
 
 // Radix digit pass: BLOCK = 512 or 1024 threads; 16 keys per thread for keys

@@ -102,25 +102,29 @@ TRTC_DEV void st_relaxed(unsigned long long* p, unsigned long long v)
 // clearing launch is needed.  The last workgroup to finish turns the global
 // histograms into exclusive digit offsets.
 //
-// LDS histograms: G = 32/KB copies per workgroup, lane l counting into copy
-// l % G, so the lanes of one atomic instruction that share a digit spread over
-// G addresses; copies are KB*256+1 words apart, so equal digits of different
-// copies sit in different banks.  (One copy per wave put up to ~22 lanes on
-// one address for keys whose bits are 1 with probability 1/8: 2^28 such keys
-// took 1.39 ms against 0.33 ms for uniform keys.)
+// LDS histograms: C counter columns per bin, lane l counting into column
+// l % C, so the 32 lanes served together by an atomic instruction never share
+// an address or a bank, whatever their digits.  (One copy per wave put up to
+// ~22 lanes on one address for keys whose bits are 1 with probability 1/8:
+// 2^28 such keys took 1.39 ms against 0.33 ms for uniform keys; 32/KB copies
+// KB*256+1 words apart, 0.29 / 0.45 ms, still spent 71% of their LDS cycles
+// on bank conflicts.)
 template <int BLOCK, int KB>
 TRTC_DEV void radix_hist(const RadixHistArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
     constexpr int PER_VEC = 16 / KB;  // keys per 16-byte load
-    constexpr int G = 32 / KB;
-    constexpr int CW = KB * 256 + 1;  // words per copy
-    __shared__ unsigned s_h[G * CW];
+    // Counters bin-major with C columns: lane l counts into column l % C, so
+    // the 32 lanes an LDS instruction serves at once hit 32 different banks
+    // whatever their digits (C = 16 for 8-byte keys: 128 KB of LDS at most).
+    constexpr int C = KB <= 4 ? 32 : 16;
+    constexpr int BINS = KB * 256;
+    __shared__ unsigned s_h[BINS * C];
     __shared__ bool s_last;
-    for (int i = threadIdx.x; i < G * CW; i += BLOCK) s_h[i] = 0;
+    for (int i = threadIdx.x; i < BINS * C; i += BLOCK) s_h[i] = 0;
     __syncthreads();
 
-    unsigned* const hcopy = s_h + (lane_id() & (unsigned)(G - 1)) * CW;
+    unsigned* const hcol = s_h + (lane_id() & (unsigned)(C - 1));
     const U flip = a.descending ? (U)~(U)0 : (U)0;
     const U* keys = (const U*)a.keys;
     const size_t stride = (size_t)gridDim.x * BLOCK;
@@ -138,7 +142,7 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
                 U k = sort_key<U>(ks[e], a.kind, flip);
 #pragma unroll
                 for (int p = 0; p < KB; p++)
-                    __hip_atomic_fetch_add(hcopy + p * 256 + ((k >> (8 * p)) & 255), 1u, __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(hcol + (p * 256 + ((k >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         };
@@ -160,7 +164,7 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
         U k = sort_key<U>(keys[i], a.kind, flip);
 #pragma unroll
         for (int p = 0; p < KB; p++)
-            __hip_atomic_fetch_add(hcopy + p * 256 + ((k >> (8 * p)) & 255), 1u, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(hcol + (p * 256 + ((k >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     {
@@ -178,10 +182,12 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     }
     __syncthreads();
 
-    for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
+    for (int i = threadIdx.x; i < BINS; i += BLOCK) {
+        // row i, columns in a lane-skewed order: neighbouring lanes read
+        // different banks at every step
         unsigned c = 0;
 #pragma unroll
-        for (int g = 0; g < G; g++) c += s_h[g * CW + i];
+        for (int j = 0; j < C; j++) c += s_h[i * C + ((j + i) & (C - 1))];
         if (c) __hip_atomic_fetch_add(a.hist + i, (unsigned long long)c, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
     }

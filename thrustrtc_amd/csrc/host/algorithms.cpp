@@ -171,14 +171,17 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     ha.done = (unsigned*)(sc + off_done);
     ha.zero_ptr = (unsigned*)(sc + off_cnt);
     ha.zero_words = (off_dstart - off_cnt) / 4;
-    // at least 64 keys per thread: every workgroup adds its KB*256 bins into
-    // the same global counters, so a grid larger than the work needs piles
-    // same-address atomics on them (2^20 keys: 48.7 us with 1024 workgroups)
-    unsigned hgrid = (unsigned)std::min<size_t>((n + (size_t)kBlock * 64 - 1) / ((size_t)kBlock * 64),
-                                                (size_t)std::max(1, context().num_cus) * 4);
+    // 1024-thread workgroups (TRTC_HIST_BLOCK), at most one per CU for their
+    // LDS, at least 16 keys per thread: every workgroup adds its KB*256 bins
+    // into the same global counters, so a grid larger than the work needs
+    // piles same-address atomics on them (2^20 keys: 48.7 us with 1024
+    // 256-thread workgroups)
+    constexpr size_t kHistBlock = 1024, kHistKeys = 16;
+    unsigned hgrid = (unsigned)std::min<size_t>((n + kHistBlock * kHistKeys - 1) / (kHistBlock * kHistKeys),
+                                                (size_t)std::max(1, context().num_cus));
     {
         void* args[] = {&ha};
-        if (!launch_kernel(hk, dim3(hgrid), dim3(kBlock), args, 0, s)) return false;
+        if (!launch_kernel(hk, dim3(hgrid), dim3((unsigned)kHistBlock), args, 0, s)) return false;
     }
 
     KernelRef pk;
