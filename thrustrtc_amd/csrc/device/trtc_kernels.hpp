@@ -195,9 +195,9 @@ __global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArg
 }
 
 template <int KB>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
+__global__ void __launch_bounds__(TRTC_HIST_BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
 {
-    trtc::seg_radix_hist<TRTC_BLOCK, KB>(a);
+    trtc::seg_radix_hist<TRTC_HIST_BLOCK, KB>(a);
 }
 
 template <class Op, class T, int ITEMS>

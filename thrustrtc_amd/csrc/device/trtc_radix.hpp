@@ -548,8 +548,8 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per
-// chunk of hist_chunk_keys keys (all KB digits from one read, per-wave LDS
-// histograms, one global add per bin and workgroup); the last chunk of a
+// chunk of hist_chunk_keys keys (all KB digits from one read, bank-conflict
+// free LDS counters as in radix_hist, one global add per bin and workgroup); the last chunk of a
 // segment to finish turns the segment's histograms into exclusive digit
 // offsets.  Counters and histograms are zeroed by the batch driver.
 template <int BLOCK, int KB>
@@ -557,14 +557,15 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
     constexpr int UNROLL = 8;
-    // lane-group copies of the LDS histograms, as in radix_hist
-    constexpr int G = 32 / KB;
-    constexpr int CW = KB * 256 + 1;
-    __shared__ unsigned s_h[G * CW];
+    // bin-major LDS counters in C columns, lane l counting into column l % C,
+    // as in radix_hist
+    constexpr int C = KB <= 4 ? 32 : 16;
+    constexpr int BINS = KB * 256;
+    __shared__ unsigned s_h[BINS * C];
     __shared__ bool s_last;
-    for (int i = threadIdx.x; i < G * CW; i += BLOCK) s_h[i] = 0;
+    for (int i = threadIdx.x; i < BINS * C; i += BLOCK) s_h[i] = 0;
     __syncthreads();
-    unsigned* const hcopy = s_h + (lane_id() & (unsigned)(G - 1)) * CW;
+    unsigned* const hcol = s_h + (lane_id() & (unsigned)(C - 1));
 
     const unsigned c = blockIdx.x;  // grid = num_hist_chunks
     const unsigned seg = a.hist_chunk[2 * c];
@@ -589,16 +590,16 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
             U sk = sort_key<U>(k[u], a.kind, flip);
 #pragma unroll
             for (int p = 0; p < KB; p++)
-                __hip_atomic_fetch_add(hcopy + p * 256 + ((sk >> (8 * p)) & 255), 1u, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(hcol + (p * 256 + ((sk >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     __syncthreads();
     unsigned* hist = a.seg_hist + (size_t)d.multi * KB * 256;
-    for (int i = threadIdx.x; i < KB * 256; i += BLOCK) {
+    for (int i = threadIdx.x; i < BINS; i += BLOCK) {
         unsigned t = 0;
 #pragma unroll
-        for (int g = 0; g < G; g++) t += s_h[g * CW + i];
+        for (int j = 0; j < C; j++) t += s_h[i * C + ((j + i) & (C - 1))];  // lane-skewed columns
         if (t) __hip_atomic_fetch_add(hist + i, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();

@@ -95,15 +95,16 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         plan.cls[c].segs.push_back(g);
         plan.keys_total += g.n;
     }
-    // histogram chunk: 8K..128K keys, small enough that the batch's multi-tile
-    // keys give every CU a few workgroups
+    // histogram chunk: 16K..128K keys (16 to 128 per thread of the 1024-thread
+    // workgroups), small enough that the batch's multi-tile keys give every
+    // CU a few workgroups
     {
         size_t multi_keys = 0;
         for (auto& C : plan.cls)
             for (auto& g : C.segs)
                 if (g.n > C.tile) multi_keys += g.n;
         size_t want = multi_keys / ((size_t)std::max(1, context().num_cus) * 4);
-        size_t ck = 8192;
+        size_t ck = 16384;
         while (ck < want && ck < 131072) ck <<= 1;
         plan.hist_chunk = ck;
     }
@@ -263,7 +264,7 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
             KernelRef hk;
             if (!get_template_kernel(seg_hist_name(plan), "", hk)) return false;
             void* args[] = {&a};
-            if (!launch_kernel(hk, dim3(C.hist_chunks), dim3(256), args, 0, s)) return false;
+            if (!launch_kernel(hk, dim3(C.hist_chunks), dim3(1024), args, 0, s)) return false;  // TRTC_HIST_BLOCK
         }
         KernelRef pk;
         if (!get_template_kernel(seg_pass_name(plan, C), "", pk)) return false;
