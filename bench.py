@@ -268,7 +268,7 @@ def leg_consumers(trtc, native, wl, a, log):
 
     def chk_copy(_):
         return (o1.to_host() == x).all()
-    entry("copy_i32", lambda: trtc.Copy(dx, o1), "trtc_map", lambda _: 8 * n, chk_copy)
+    entry("copy_i32", lambda: trtc.Copy(dx, o1), "trtc_copy_bytes16", lambda _: 8 * n, chk_copy)
     of = trtc.device_vector("float", n)
 
     def chk_copy_f32(_):

@@ -175,3 +175,25 @@ def test_scan_8byte_tiles(trtc, n):
     of = trtc.device_vector("double", n)
     trtc.Inclusive_Scan(df, of)
     assert np.array_equal(of.to_host(), np.cumsum(f))
+
+
+@pytest.mark.parametrize("n", [3, 17, 4099, (1 << 20) + 5])
+def test_copy_offsets_and_short_output(trtc, n):
+    """Same-type contiguous copies: 16-byte aligned ends run the 16-byte copy
+    kernel (with a byte tail), ranges starting mid-vector the map kernel (4 and
+    8 bytes) or hipMemcpyAsync (1 byte); an output shorter than the input is
+    an error for every path, with nothing written."""
+    say(f"copy offsets n={n}")
+    rng = np.random.default_rng(n + 5)
+    for dt, ct in [(np.int32, "int32_t"), (np.uint8, "uint8_t"), (np.float64, "double")]:
+        x = rng.integers(0, 100, size=n + 1).astype(dt)
+        d = trtc.device_vector_from_numpy(x)
+        o = trtc.device_vector(ct, n + 1)
+        trtc.Copy(trtc.DVRange(d, 1, n + 1), trtc.DVRange(o, 0, n))  # source off by one item
+        assert np.array_equal(o.to_host(0, n), x[1:])
+        trtc.Copy(trtc.DVRange(d, 0, n), trtc.DVRange(o, 1, n + 1))  # destination off by one item
+        assert np.array_equal(o.to_host(1, n + 1), x[:n])
+        keep = o.to_host()
+        with pytest.raises(SystemError):
+            trtc.Copy(d, trtc.DVRange(o, 0, n))
+        assert np.array_equal(o.to_host(), keep)

@@ -39,6 +39,21 @@ trtc_gather_bytes(const unsigned char* src, unsigned char* dst, const unsigned* 
     }
 }
 
+// Contiguous copy of nbytes between 16-byte aligned buffers: 16-byte vectors
+// grid-stride (one per thread when the grid covers them all, as the host
+// launches it), then the last nbytes % 16 bytes by the first threads of
+// workgroup 0.
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK)
+trtc_copy_bytes16(const unsigned char* src, unsigned char* dst, size_t nbytes)
+{
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const size_t nvec = nbytes / 16;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride)
+        ((v4u*)dst)[i] = __builtin_nontemporal_load((const v4u*)src + i);
+    if (blockIdx.x == 0 && threadIdx.x < (nbytes & 15)) dst[nvec * 16 + threadIdx.x] = src[nvec * 16 + threadIdx.x];
+}
+
 // dst[i] = i (the payload of a keys-with-index sort)
 extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_iota_u32(unsigned* dst, size_t n)
 {

@@ -811,6 +811,19 @@ bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s)
     return launch_kernel(k, dim3(grid_for(words, 256)), dim3(256), args, 0, s);
 }
 
+bool copy_bytes(const void* src, void* dst, size_t nbytes, hipStream_t s)
+{
+    if (!nbytes) return true;
+    if ((((size_t)src | (size_t)dst) & 15) != 0)
+        return check_hip(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+    KernelRef k;
+    if (!get_builtin("trtc_copy_bytes16", k)) return false;
+    void* args[] = {&src, &dst, &nbytes};
+    // one vector per thread (a grid-strided loop keeps 16-byte copies ~10% below this)
+    const size_t g = std::min<size_t>(std::max<size_t>((nbytes / 16 + 255) / 256, 1), (size_t)1 << 30);
+    return launch_kernel(k, dim3((unsigned)g), dim3(256), args, 0, s);
+}
+
 bool iota_u32(unsigned* p, size_t n, hipStream_t s)
 {
     if (!n) return true;

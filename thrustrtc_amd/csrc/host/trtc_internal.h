@@ -143,6 +143,9 @@ void pinned_release(void* p, hipStream_t s);
 
 // Built-in utility kernels.
 bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s);
+// Device-to-device copy of nbytes: a 16-byte-per-thread kernel when both
+// pointers are 16-byte aligned, hipMemcpyAsync otherwise.
+bool copy_bytes(const void* src, void* dst, size_t nbytes, hipStream_t s);
 bool iota_u32(unsigned* p, size_t n, hipStream_t s);
 bool gather_bytes(const void* src, void* dst, const unsigned* idx, size_t n, size_t elem, hipStream_t s);
 
