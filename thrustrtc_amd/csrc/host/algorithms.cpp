@@ -208,10 +208,10 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kRadixBlock), args, 0, s)) return false;
     }
     if (KB & 1) {
-        if (!check_hip(hipMemcpyAsync(keys, ktmp.p, n * KB, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+        if (!copy_bytes(ktmp.p, keys, n * KB, s))
             return false;
         if (val_bytes &&
-            !check_hip(hipMemcpyAsync(vals, vtmp.p, n * val_bytes, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+            !copy_bytes(vtmp.p, vals, n * val_bytes, s))
             return false;
     }
     return true;
@@ -277,12 +277,12 @@ bool merge_sort(void* keys, const std::string& T, size_t esize, size_t n, const 
         }
     }
     if (cur == 1 &&
-        !check_hip(hipMemcpyAsync(keys, kb.p, n * esize, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+        !copy_bytes(kb.p, keys, n * esize, s))
         return false;
     if (!with_idx) return true;
     PoolBuf vcopy(n * vsize, s);
     if (!vcopy.p) return false;
-    if (!check_hip(hipMemcpyAsync(vcopy.p, vals, n * vsize, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+    if (!copy_bytes(vals, vcopy.p, n * vsize, s))
         return false;
     return gather_bytes(vcopy.p, vals, (const unsigned*)ibuf[cur], n, vsize, s);
 }
@@ -343,7 +343,7 @@ bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp
         if (!idx.p || !vcopy.p) return false;
         if (!iota_u32(idx.as<unsigned>(), n, s)) return false;
         if (!radix_sort(kp, idx.p, n, kb, kind, desc, 4, s)) return false;
-        if (!check_hip(hipMemcpyAsync(vcopy.p, vp, n * vb, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
+        if (!copy_bytes(vp, vcopy.p, n * vb, s))
             return false;
         return gather_bytes(vcopy.p, vp, idx.as<unsigned>(), n, vb, s);
     }
