@@ -197,3 +197,27 @@ def test_copy_offsets_and_short_output(trtc, n):
         with pytest.raises(SystemError):
             trtc.Copy(d, trtc.DVRange(o, 0, n))
         assert np.array_equal(o.to_host(), keep)
+
+
+@pytest.mark.parametrize("n", [1, 2047, 2049, 100003, (1 << 22) + 7])
+def test_fill_sequence_tabulate_sizes(trtc, n):
+    """Fill, Sequence (three forms) and Tabulate on the element-wise map kernel
+    (8 indices per thread in blocks of 2048): sizes around the block, int8
+    wrap-around and doubles."""
+    say(f"fill/sequence/tabulate n={n}")
+    d = trtc.device_vector("int32_t", n)
+    trtc.Fill(d, trtc.DVInt32(-4))
+    assert (d.to_host() == -4).all()
+    trtc.Sequence(d)
+    assert np.array_equal(d.to_host(), np.arange(n, dtype=np.int32))
+    trtc.Sequence(d, trtc.DVInt32(5), trtc.DVInt32(-3))
+    want = (5 - 3 * np.arange(n, dtype=np.int64)).astype(np.int32)
+    assert np.array_equal(d.to_host(), want)
+    trtc.Tabulate(d, trtc.Negate())
+    assert np.array_equal(d.to_host(), (-want.astype(np.int64)).astype(np.int32))
+    b = trtc.device_vector("int8_t", n)
+    trtc.Sequence(b, trtc.DVInt8(100))
+    assert np.array_equal(b.to_host(), (100 + np.arange(n)).astype(np.int8))  # wraps as the reference's C++ does
+    f = trtc.device_vector("double", n)
+    trtc.Sequence(f, trtc.DVDouble(0.5), trtc.DVDouble(0.25))
+    assert np.array_equal(f.to_host(), 0.5 + np.arange(n, dtype=np.float64) * 0.25)

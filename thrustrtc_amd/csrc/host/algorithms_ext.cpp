@@ -152,8 +152,8 @@ bool run_map(const std::vector<const char*>& names, const char* value, const cha
 // ================================================================ element-wise
 bool TRTC_Fill(DVVectorLike& vec, const DeviceViewable& value)
 {
-    static TRTC_For f({"view_vec", "value"}, "idx", "    view_vec[idx] = (decltype(view_vec)::value_t)value;\n");
-    return run_for(f, vec.size(), {&vec, &value});
+    return run_map({"view_vec", "value"}, "(decltype(view_vec)::value_t)value", "view_vec[idx] = _x;", vec.size(),
+                   {&vec, &value});
 }
 
 bool TRTC_Transform(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& op)
@@ -281,31 +281,28 @@ bool TRTC_Adjacent_Difference(const DVVectorLike& vec_in, DVVectorLike& vec_out)
 
 bool TRTC_Sequence(DVVectorLike& vec)
 {
-    static TRTC_For f({"v"}, "idx", "    v[idx] = (decltype(v)::value_t)idx;\n");
-    return run_for(f, vec.size(), {&vec});
+    return run_map({"v"}, "(decltype(v)::value_t)idx", "v[idx] = _x;", vec.size(), {&vec});
 }
 
 bool TRTC_Sequence(DVVectorLike& vec, const DeviceViewable& value_init)
 {
-    static TRTC_For f({"v", "value_init"}, "idx",
-                      "    v[idx] = (decltype(v)::value_t)value_init + (decltype(v)::value_t)idx;\n");
-    return run_for(f, vec.size(), {&vec, &value_init});
+    return run_map({"v", "value_init"}, "(decltype(v)::value_t)((decltype(v)::value_t)value_init + (decltype(v)::value_t)idx)",
+                   "v[idx] = _x;", vec.size(), {&vec, &value_init});
 }
 
 bool TRTC_Sequence(DVVectorLike& vec, const DeviceViewable& value_init, const DeviceViewable& value_step)
 {
-    static TRTC_For f({"v", "value_init", "value_step"}, "idx",
-                      "    v[idx] = (decltype(v)::value_t)value_init +\n"
-                      "             (decltype(v)::value_t)idx * (decltype(v)::value_t)value_step;\n");
-    return run_for(f, vec.size(), {&vec, &value_init, &value_step});
+    return run_map({"v", "value_init", "value_step"},
+                   "(decltype(v)::value_t)((decltype(v)::value_t)value_init +\n"
+                   "             (decltype(v)::value_t)idx * (decltype(v)::value_t)value_step)",
+                   "v[idx] = _x;", vec.size(), {&vec, &value_init, &value_step});
 }
 
 // The reference's tabulate.cpp applies op to each element (vec[i] =
 // op(vec[i])); for a vector holding 0..n-1 that is Thrust's op(i).
 bool TRTC_Tabulate(DVVectorLike& vec, const Functor& op)
 {
-    static TRTC_For f({"v", "op"}, "idx", "    v[idx] = op(v[idx]);\n");
-    return run_for(f, vec.size(), {&vec, &op});
+    return run_map({"v", "op"}, "(decltype(v)::value_t)op(v[idx])", "v[idx] = _x;", vec.size(), {&vec, &op});
 }
 
 bool TRTC_Gather(const DVVectorLike& vec_map, const DVVectorLike& vec_in, DVVectorLike& vec_out)
