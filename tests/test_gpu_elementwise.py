@@ -221,3 +221,28 @@ def test_fill_sequence_tabulate_sizes(trtc, n):
     f = trtc.device_vector("double", n)
     trtc.Sequence(f, trtc.DVDouble(0.5), trtc.DVDouble(0.25))
     assert np.array_equal(f.to_host(), 0.5 + np.arange(n, dtype=np.float64) * 0.25)
+
+
+@pytest.mark.parametrize("n", [1, 2, 2048, 2049, 100003])
+def test_replace_copy_and_adjacent_difference_sizes(trtc, oa, n):
+    """Replace_Copy / Replace_Copy_If / Adjacent_Difference on the element-wise
+    map kernel: sizes around the 2048-index block, the in-place difference
+    (which reads a copy of the input) and a user operator."""
+    say(f"replace_copy / adjacent_difference n={n}")
+    rng = np.random.default_rng(n + 17)
+    x = rng.integers(-5, 5, size=n, dtype=np.int32)
+    d = trtc.device_vector_from_numpy(x)
+    o = trtc.device_vector("int32_t", n)
+    trtc.Replace_Copy(d, o, trtc.DVInt32(3), trtc.DVInt32(-100))
+    assert np.array_equal(o.to_host(), np.where(x == 3, -100, x))
+    neg = trtc.Functor({}, ["x"], "        return x < 0;\n")
+    trtc.Replace_Copy_If(d, o, neg, trtc.DVInt32(0))
+    assert np.array_equal(o.to_host(), np.where(x < 0, 0, x))
+    trtc.Adjacent_Difference(d, o)
+    assert np.array_equal(o.to_host(), oa.adjacent_difference_np(x))
+    trtc.Adjacent_Difference(d, o, trtc.Plus())
+    want = x.copy()
+    want[1:] = x[1:] + x[:-1]
+    assert np.array_equal(o.to_host(), want)
+    trtc.Adjacent_Difference(d, d)  # in place
+    assert np.array_equal(d.to_host(), oa.adjacent_difference_np(x))

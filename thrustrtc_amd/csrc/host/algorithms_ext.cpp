@@ -223,22 +223,20 @@ bool TRTC_Replace_If(DVVectorLike& vec, const Functor& pred, const DeviceViewabl
 bool TRTC_Replace_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out, const DeviceViewable& old_value,
                        const DeviceViewable& new_value)
 {
-    static TRTC_For f({"vin", "vout", "old_value", "new_value"}, "idx",
-                      "    auto x = vin[idx];\n"
-                      "    vout[idx] = x == (decltype(vin)::value_t)old_value ? (decltype(vout)::value_t)new_value\n"
-                      "                                                       : (decltype(vout)::value_t)x;\n");
     if (!need(vec_out, vec_in.size(), "Replace_Copy", "vec_out")) return false;
-    return run_for(f, vec_in.size(), {&vec_in, &vec_out, &old_value, &new_value});
+    return run_map({"vin", "vout", "old_value", "new_value"},
+                   "vin[idx] == (decltype(vin)::value_t)old_value ? (decltype(vout)::value_t)new_value\n"
+                   "                                                : (decltype(vout)::value_t)vin[idx]",
+                   "vout[idx] = _x;", vec_in.size(), {&vec_in, &vec_out, &old_value, &new_value});
 }
 
 bool TRTC_Replace_Copy_If(const DVVectorLike& vec_in, DVVectorLike& vec_out, const Functor& pred,
                           const DeviceViewable& new_value)
 {
-    static TRTC_For f({"vin", "vout", "pred", "new_value"}, "idx",
-                      "    auto x = vin[idx];\n"
-                      "    vout[idx] = pred(x) ? (decltype(vout)::value_t)new_value : (decltype(vout)::value_t)x;\n");
     if (!need(vec_out, vec_in.size(), "Replace_Copy_If", "vec_out")) return false;
-    return run_for(f, vec_in.size(), {&vec_in, &vec_out, &pred, &new_value});
+    return run_map({"vin", "vout", "pred", "new_value"},
+                   "pred(vin[idx]) ? (decltype(vout)::value_t)new_value : (decltype(vout)::value_t)vin[idx]",
+                   "vout[idx] = _x;", vec_in.size(), {&vec_in, &vec_out, &pred, &new_value});
 }
 
 bool TRTC_For_Each(DVVectorLike& vec, const Functor& f)
@@ -256,10 +254,13 @@ bool TRTC_Adjacent_Difference(const DVVectorLike& vec_in, DVVectorLike& vec_out,
     size_t n = vec_in.size();
     if (n == 0) return true;
     if (!need(vec_out, n, "Adjacent_Difference", "vec_out")) return false;
-    static TRTC_For f({"vin", "vout", "op"}, "idx",
-                      "    auto x = vin[idx];\n"
-                      "    if (idx > 0) x = (decltype(vin)::value_t)op(x, vin[idx - 1]);\n"
-                      "    vout[idx] = (decltype(vout)::value_t)x;\n");
+    // the element-wise map: out[idx] from in[idx] and in[idx - 1]
+    auto run = [&](const DVVectorLike& src) {
+        return run_map({"vin", "vout", "op"},
+                       "(decltype(vout)::value_t)(idx > 0 ? (decltype(vin)::value_t)op(vin[idx], vin[idx - 1])\n"
+                       "                                    : vin[idx])",
+                       "vout[idx] = _x;", n, {&src, &vec_out, &binary_op});
+    };
     void* pi = contiguous_data(vec_in);
     if (pi && pi == contiguous_data(vec_out)) {
         hipStream_t s = cur_stream();
@@ -268,10 +269,10 @@ bool TRTC_Adjacent_Difference(const DVVectorLike& vec_in, DVVectorLike& vec_out,
         DVVectorAdaptor cpy(vec_in.name_elem_cls().c_str(), n, tmp.p);
         if (!check_hip(hipMemcpyAsync(tmp.p, pi, n * vec_in.elem_size(), hipMemcpyDeviceToDevice, s), "hipMemcpyAsync"))
             return false;
-        if (!run_for(f, n, {&cpy, &vec_out, &binary_op})) return false;
+        if (!run(cpy)) return false;
         return check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
-    return run_for(f, n, {&vec_in, &vec_out, &binary_op});
+    return run(vec_in);
 }
 
 bool TRTC_Adjacent_Difference(const DVVectorLike& vec_in, DVVectorLike& vec_out)
