@@ -407,18 +407,6 @@ double or_sum_f32_fp64(const float* in, uint64_t n)
     for (uint64_t i = 0; i < n; i++) s += (double)in[i];
     return s;
 }
-double or_sum_f64_fp64(const double* in, uint64_t n)
-{
-    // Neumaier-compensated sum in double
-    double s = 0.0, c = 0.0;
-    for (uint64_t i = 0; i < n; i++) {
-        double t = s + in[i];
-        if (std::fabs(s) >= std::fabs(in[i])) c += (s - t) + in[i];
-        else c += (in[i] - t) + s;
-        s = t;
-    }
-    return s + c;
-}
 
 // The reference's exact float reduction order (general_reduce.cpp:30-95): blocks
 // of 256 consecutive elements, each reduced by the halving tree

@@ -5,9 +5,16 @@ reduce and copy_if).  GPU tests compare the HIP results against these hashes,
 so the full-size parity check needs neither the oracle's run time nor the
 reference on the machine that runs them.  Run from the repository root:
 
-    python tests/golden/make_large_hashes.py
+    python tests/golden/make_large_hashes.py            # large_hashes.json
+    python tests/golden/make_large_hashes.py --batched  # batched_segments_hashes.json
 
-Takes several minutes on 8 host cores (the 2^28 stable sorts dominate).
+The first takes several minutes on 8 host cores (the 2^28 stable sorts
+dominate).  The second writes, for the batched-segments workload (4096
+segments, sizes log-uniform in [16, 2^20] from mt19937_64 seed 7, uint32 keys
+uniform from mt19937 seed 2 with the uint32 index as payload, int32 values
+uniform in [-1000, 1000] from mt19937 seed 5), the hash of every segment's
+stable Sort_By_Key keys and payload and of its Inclusive_Scan, each segment
+computed alone by the oracle.
 """
 import json
 import os
@@ -67,5 +74,28 @@ def main():
         json.dump(res, fo, indent=1, sort_keys=True)
 
 
+def batched():
+    t0 = time.time()
+    sizes, total = wl.segment_sizes(4096, 7, 16, 1 << 20)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    keys = wl.u32_uniform(total, 2)
+    x = wl.i32_range(total, 5, -1000, 1000)
+    segs = []
+    for s in range(len(sizes)):
+        lo, hi = int(offs[s]), int(offs[s + 1])
+        perm = orc.sort_perm(keys[lo:hi], "uint32_t")
+        segs.append(" ".join([h(keys[lo:hi][perm]), h((perm + lo).astype(np.uint32)),
+                              h(orc.inclusive_scan(x[lo:hi], "int32_t"))]))
+    res = {"segments": len(sizes), "total": total, "sizes": h(sizes), "keys_in": h(keys), "values_in": h(x),
+           "per_segment": "keys_out values_out(global index payload) inclusive_scan_out",
+           "hashes": segs}
+    print("batched segments", time.time() - t0, flush=True)
+    with open(os.path.join(HERE, "batched_segments_hashes.json"), "w") as fo:
+        json.dump(res, fo, indent=0, separators=(",", ": "))
+
+
 if __name__ == "__main__":
-    main()
+    if "--batched" in sys.argv[1:]:
+        batched()
+    else:
+        main()

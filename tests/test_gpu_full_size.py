@@ -180,3 +180,53 @@ def check_sort(trtc, n, kdt, vdt, greater):
         trtc.Sort_By_Key(dk, dv, comp)
         assert np.array_equal(dk.to_host().view(np.uint8), k[perm].view(np.uint8))
         assert np.array_equal(dv.to_host(), v[perm])
+
+
+BATCHED = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "batched_segments_hashes.json")))
+
+
+@pytest.mark.parametrize("schedule", [None, "<=4096:256x16 <=262144:512x16 rest:1024x16"])
+def test_batched_segments_4096_workload(trtc, wl, schedule):
+    """The batched-segments workload at its full size: 4096 segments with
+    sizes log-uniform in [16, 2^20] (mt19937_64 seed 7), uint32 keys (seed 2)
+    with the uint32 index payload Sort_By_Key and int32 values in
+    [-1000, 1000] (seed 5) Inclusive_Scan over the same segments, recorded in
+    one Batch and replayed as one graph launch; every segment's outputs
+    against the oracle's hashes of that segment sorted / scanned alone.  Run
+    autotuned and with the 1024-thread class forced on segments above 2^18
+    keys."""
+    say(f"batched segments 4096, schedule={schedule or 'autotuned'}")
+    g = BATCHED
+    trtc.Set_Batch_Autotune(True)
+    trtc.Set_Batch_Schedule(schedule)
+    try:
+        sizes, total = wl.segment_sizes(g["segments"], 7, 16, 1 << 20)
+        assert total == g["total"] and h(wl, sizes) == g["sizes"]
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        keys = wl.u32_uniform(total, 2)
+        assert h(wl, keys) == g["keys_in"]
+        x = wl.i32_range(total, 5, -1000, 1000)
+        assert h(wl, x) == g["values_in"]
+        dk = trtc.device_vector_from_numpy(keys)
+        dv = trtc.device_vector_from_numpy(np.arange(total, dtype=np.uint32))
+        dx = trtc.device_vector_from_numpy(x)
+        dy = trtc.device_vector("int32_t", total)
+        del keys, x
+        b = trtc.Batch()
+        b.add_sort_by_key(dk, dv, offs)
+        b.add_inclusive_scan(dx, dy, offs)
+        b.instantiate()
+        b.launch()
+        trtc.Wait()
+        if schedule is None:
+            say(f"  tuner chose: {trtc.Batch_Tuning_Report()[-1]['chosen']}")
+        ok, ov, oy = dk.to_host(), dv.to_host(), dy.to_host()
+        bad = []
+        for s in range(len(sizes)):
+            lo, hi = int(offs[s]), int(offs[s + 1])
+            got = " ".join([h(wl, ok[lo:hi]), h(wl, ov[lo:hi]), h(wl, oy[lo:hi])])
+            if got != g["hashes"][s]:
+                bad.append((s, int(sizes[s])))
+        assert not bad, f"{len(bad)} segments differ from the oracle, first: {bad[:5]}"
+    finally:
+        trtc.Set_Batch_Schedule(None)

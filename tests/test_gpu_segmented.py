@@ -13,8 +13,11 @@ from helpers import Oracle, workloads, NP
 pytestmark = pytest.mark.gpu
 
 # every schedule the batch autotuner may pick for 4-byte and 8-byte keys
-SCHEDULES_4 = ["rest:512x16", "rest:512x8", "rest:256x16", "<=2048:256x8 rest:512x16", "<=4096:256x16 rest:512x16"]
-SCHEDULES_8 = ["rest:512x8", "rest:256x8", "<=2048:256x8 rest:512x8"]
+# (test_tuner_candidates_are_all_tested checks this against the tuner's own
+# candidate list)
+SCHEDULES_4 = ["rest:512x16", "rest:512x8", "rest:256x16", "<=2048:256x8 rest:512x16", "<=4096:256x16 rest:512x16",
+               "<=2048:256x8 <=262144:512x16 rest:1024x16", "<=4096:256x16 <=262144:512x16 rest:1024x16"]
+SCHEDULES_8 = ["rest:512x8", "rest:256x8", "<=2048:256x8 rest:512x8", "<=2048:256x8 <=262144:512x8 rest:1024x8"]
 
 
 @pytest.fixture(scope="module")
@@ -42,6 +45,17 @@ def edge_sizes():
     # empty, single, tile boundaries of every schedule, multi-tile segments
     return [0, 1, 2, 3, 63, 64, 65, 511, 512, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 0,
             16383, 16384, 16385, 30000, 100003, 5, 262144 + 17]
+
+
+def large_edge_sizes():
+    # segments on both sides of the 2^18 class boundary of the 1024-thread
+    # schedules, and 1024x16 (16384-key) / 1024x8 (8192-key) tile edges
+    return [262143, 262144, 262145, 262144 + 8191, 262144 + 8193, 262144 + 16383, 262144 + 16385, 1 << 20,
+            (1 << 20) + 1]
+
+
+def sizes_for(schedule):
+    return edge_sizes() + (large_edge_sizes() if "1024x" in schedule else [])
 
 
 def offsets_of(sizes):
@@ -82,7 +96,7 @@ def check_segments_sorted(orc, keys, out_k, out_v, offs, elem, greater, vals_dty
 @pytest.mark.parametrize("greater", [False, True])
 def test_segmented_sort_by_key_4byte(trtc, orc, wl, schedule, kind, greater):
     say(f"segmented sort_by_key {kind} greater={greater} schedule={schedule}")
-    sizes = edge_sizes()
+    sizes = sizes_for(schedule)
     offs = offsets_of(sizes)
     n = int(offs[-1])
     keys, elem = keys_for(wl, kind, n, 11)
@@ -97,7 +111,7 @@ def test_segmented_sort_by_key_4byte(trtc, orc, wl, schedule, kind, greater):
 @pytest.mark.parametrize("kind", ["u64", "f64"])
 def test_segmented_sort_by_key_8byte(trtc, orc, wl, schedule, kind):
     say(f"segmented sort_by_key {kind} schedule={schedule}")
-    sizes = edge_sizes()
+    sizes = sizes_for(schedule)
     offs = offsets_of(sizes)
     n = int(offs[-1])
     keys, elem = keys_for(wl, kind, n, 21)
@@ -108,10 +122,11 @@ def test_segmented_sort_by_key_8byte(trtc, orc, wl, schedule, kind):
     check_segments_sorted(orc, keys, dk.to_host(), dv.to_host(), offs, elem, False, np.uint64)
 
 
-@pytest.mark.parametrize("schedule", ["rest:512x16", "<=2048:256x8 rest:512x16"])
+@pytest.mark.parametrize("schedule", ["rest:512x16", "<=2048:256x8 rest:512x16",
+                                      "<=2048:256x8 <=262144:512x16 rest:1024x16"])
 def test_segmented_sort_keys_only(trtc, orc, wl, schedule):
     say(f"segmented keys-only sort schedule={schedule}")
-    offs = offsets_of(edge_sizes())
+    offs = offsets_of(sizes_for(schedule))
     n = int(offs[-1])
     keys, elem = keys_for(wl, "u32_low", n, 31)
     dk = trtc.device_vector_from_numpy(keys)
@@ -164,6 +179,25 @@ def test_segmented_log_uniform_autotuned(trtc, orc, wl):
     check_segments_sorted(orc, keys, dk.to_host(), dv.to_host(), offs, "uint32_t", False)
     rep = trtc.Batch_Tuning_Report()
     assert rep and rep[-1]["chosen"] in [c["schedule"] for c in rep[-1]["candidates"]]
+
+
+@pytest.mark.parametrize("vals", ["uint32_t", "uint64_t"])
+def test_tuner_candidates_are_all_tested(trtc, wl, vals):
+    """Every schedule the autotuner may choose is one of the schedules the
+    parity tests above run (a new candidate must come with its parity test)."""
+    say(f"autotuner candidates, {vals} values")
+    trtc.Set_Batch_Schedule(None)
+    trtc.Set_Batch_Autotune(True)
+    sizes, total = wl.segment_sizes(96, 23, 16, 1 << 19)
+    offs = offsets_of(sizes)
+    dk = trtc.device_vector_from_numpy(wl.u32_uniform(total, 3))
+    dv = trtc.device_vector(vals, total)
+    trtc.Segmented_Sort_By_Key(dk, dv, offs)
+    rep = trtc.Batch_Tuning_Report()
+    assert rep
+    tested = SCHEDULES_4 if vals == "uint32_t" else SCHEDULES_8
+    for c in rep[-1]["candidates"]:
+        assert c["schedule"] in tested, c["schedule"]
 
 
 # ---------------------------------------------------------------- scan
