@@ -261,6 +261,16 @@ TRTC_CAPI void* n_dvvectorlike_data(void* cptr);
  * (e.g. "gfx950").  0 = compiled, -1 = compile error (log on stderr).
  * Callers: offline cache warm-up and build-time checks of functor code. */
 TRTC_CAPI int n_jit_compile_check(const char* arch, const char* name_expr);
+/* Code-object cache of run-time compiled kernels (in memory, and on disk
+ * under $TRTC_CACHE_DIR/<arch>/<key>.co, else $XDG_CACHE_HOME/thrustrtc_amd
+ * or ~/.cache/thrustrtc_amd): counts of in-memory hits, disk hits, hipRTC
+ * compilations and disk entries rejected as damaged (out[0..3]).
+ * Callers: cache tests and deployment checks that a warm cache is used. */
+TRTC_CAPI void n_jit_cache_stats(unsigned long long* out4);
+/* 1 if `blob` (size bytes) is a well-formed disk-cache entry (magic, code
+ * length and checksum match), else 0; host only, needs no device.  The loader
+ * sees only entries that pass this check. */
+TRTC_CAPI int n_jit_cache_entry_valid(const char* blob, unsigned long long size);
 /* Kernel timing: between n_profile_begin and n_profile_end every kernel
  * launch is bracketed by HIP events on its stream; n_profile_end synchronises
  * and writes {"<kernel>": {"count": c, "total_ms": t}, ...} into json_out

@@ -148,3 +148,35 @@ def test_merge_sort_kernels_compile_for_user_comparator(nm):
     for k in ("trtc_msort_tile_k", "trtc_msort_merge_k", "trtc_msort_split_k"):
         assert n.n_jit_compile_check(b"gfx950", f"{k}<uint32_t, {name}>".encode()) == 0
     assert n.n_jit_compile_check(b"gfx950", b"trtc_msort_merge_k<double, Greater>") == 0
+
+
+def _entry(lowered, code):
+    """A disk-cache entry as the library writes it (the format is part of the
+    cache's contract: header with code length and a 64-bit FNV-1a checksum
+    whose offset basis is the library's 1469598103934665603)."""
+    h = 1469598103934665603
+    for b in code:
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return b"TRTCCO2 %d %016x %s\n" % (len(code), h, lowered) + code
+
+
+def test_cache_entry_validator(nm):
+    """The host check every disk-cache entry passes before the loader sees it:
+    a good entry passes; truncated, extended, bit-flipped, empty, name-only,
+    old-format and foreign files are rejected."""
+    n = nm.native
+    code = bytes(range(256)) * 7 + b"\x7fELF tail"
+    good = _entry(b"_Z3fooPi", code)
+    ok = lambda blob: n.n_jit_cache_entry_valid(blob, len(blob))
+    assert ok(good) == 1
+    assert ok(good[:-1]) == 0                         # truncated code
+    assert ok(good + b"x") == 0                       # trailing bytes
+    flipped = bytearray(good)
+    flipped[-100] ^= 0x10
+    assert ok(bytes(flipped)) == 0                    # damaged code
+    assert ok(b"") == 0
+    assert ok(b"TRTCCO2 0 0000000000000000 name\n") == 0   # name only
+    assert ok(b"_Z3fooPi\n" + code) == 0               # earlier format (name line + code)
+    assert ok(b"\x7fELF" + code) == 0                  # foreign file
+    assert ok(_entry(b"", code)) == 0                  # no kernel name
+    assert ok(good.replace(b"TRTCCO2 %d" % len(code), b"TRTCCO2 %d" % (len(code) + 1))) == 0

@@ -3,6 +3,7 @@
   libPyThrustRTC.so     the runtime + C ABI (host C++ compiled by g++, the
                         ahead-of-time gfx950 kernels compiled by hipcc, linked
                         by hipcc against libamdhip64 and libhiprtc)
+  libThrustRTC.so       symbolic link to it: the C++ API's library name
   libtrtc_workloads.so  deterministic synthetic-input generators (host only)
 
 The device headers under csrc/device/ are also embedded into the runtime as
@@ -126,6 +127,12 @@ def build(jobs=None, verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
           f"-L{ROCM}/lib", "-lhiprtc", "-Wl,-rpath," + f"{ROCM}/lib", "-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
+    # the reference's C++ library name (cpp/CMakeLists.txt:158): the same
+    # library, so C++ callers link -lThrustRTC unchanged
+    alias = os.path.join(PKG, "libThrustRTC.so")
+    if os.path.islink(alias) or os.path.exists(alias):
+        os.remove(alias)
+    os.symlink(os.path.basename(LIB), alias)
     return LIB
 
 

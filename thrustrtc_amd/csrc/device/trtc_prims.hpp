@@ -164,7 +164,11 @@ template <class T, class Op> TRTC_DEV T wave_reverse_reduce(T x, unsigned last, 
 // inclusive prefix ends the walk; otherwise its aggregate is folded in and
 // the window moves back by 64.  Every tile publishes its aggregate before it
 // looks back and tile ids are handed out in launch order by an atomic counter,
-// so every awaited tile is already resident and the wait ends.
+// so every awaited tile is already resident and the wait ends.  The sequence's
+// first tile publishes only an inclusive prefix, so the walk ends there at the
+// latest; should it ever pass that tile (status left over from another launch
+// in memory that was not cleared), it stops with what it folded instead of
+// walking on forever.
 template <class T, class Op>
 TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op, size_t first = 0)
 {
@@ -200,6 +204,7 @@ TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op, size_t first = 
                 continue;
             }
             unsigned long long in_window = ballot(st != 3);
+            if (!in_window) break;  // the window lies wholly before the first tile
             last = 63 - (unsigned)__builtin_clzll(in_window);
         }
         T w = wave_reverse_reduce(v, last, op);

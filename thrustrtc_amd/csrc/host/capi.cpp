@@ -886,6 +886,18 @@ int n_jit_compile_check(const char* arch, const char* name_expr)
     return -1;
 }
 
+void n_jit_cache_stats(unsigned long long* out4)
+{
+    if (out4) jit_cache_stats(out4);
+}
+
+int n_jit_cache_entry_valid(const char* blob, unsigned long long size)
+{
+    if (!blob) return 0;
+    std::string lowered, code;
+    return parse_cache_entry(std::string(blob, (size_t)size), lowered, code) ? 1 : 0;
+}
+
 int n_profile_begin(void)
 {
     profile_begin();

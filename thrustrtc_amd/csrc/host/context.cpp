@@ -14,6 +14,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -42,6 +43,7 @@ bool check_hip(hipError_t e, const char* what)
 {
     if (e == hipSuccess) return true;
     TRTC_ERR("%s failed: %s (%d)", what, hipGetErrorString(e), (int)e);
+    (void)hipGetLastError();  // reported here; later checks must not see it again
     return false;
 }
 
@@ -462,22 +464,93 @@ bool rtc_compile(const std::string& src, const std::string& name_expr, const std
     return true;
 }
 
-// Cache file format: "<lowered name>\n" followed by the code object bytes.
-bool jit_get(const std::string& src, const std::string& name_expr, const std::string& fn_name, KernelRef& out)
+// Hash of the embedded device headers, computed once: every JIT key covers
+// them, and hashing their ~136 KB on each lookup cost ~0.2 ms per call.
+uint64_t embedded_sources_hash()
+{
+    static const uint64_t h = [] {
+        uint64_t x = fnv1a("trtc-embedded", 13);
+        for (int i = 0; i < k_embedded_count; i++) {
+            x = fnv1a(k_embedded_names[i], strlen(k_embedded_names[i]), x);
+            x = fnv1a(k_embedded_sources[i], strlen(k_embedded_sources[i]), x);
+        }
+        return x;
+    }();
+    return h;
+}
+
+struct JitStats
+{
+    std::atomic<unsigned long long> mem_hits{0}, disk_hits{0}, compiles{0}, rejected{0};
+};
+JitStats& jit_stats()
+{
+    static JitStats* s = new JitStats();
+    return *s;
+}
+
+// Disk cache entry: "TRTCCO2 <code bytes> <fnv1a-64 of the code, hex> <lowered name>\n" followed by the
+// code object.  compile + write = make_cache_entry; read = parse_cache_entry, which checks the length and
+// the checksum on the host so that a truncated, damaged or foreign file never reaches hipModuleLoadData.
+std::string make_cache_entry(const std::string& lowered, const std::string& code)
+{
+    char head[96];
+    snprintf(head, sizeof(head), "TRTCCO2 %zu %016llx ", code.size(),
+             (unsigned long long)fnv1a(code.data(), code.size()));
+    return head + lowered + "\n" + code;
+}
+}  // namespace
+
+bool parse_cache_entry(const std::string& blob, std::string& lowered, std::string& code)
+{
+    static const char magic[] = "TRTCCO2 ";
+    if (blob.size() < sizeof(magic) - 1 || blob.compare(0, sizeof(magic) - 1, magic) != 0) return false;
+    size_t nl = blob.find('\n');
+    if (nl == std::string::npos || nl > 4096) return false;
+    const std::string head = blob.substr(sizeof(magic) - 1, nl - (sizeof(magic) - 1));
+    unsigned long long len = 0, sum = 0;
+    int used = 0;
+    if (sscanf(head.c_str(), "%llu %16llx %n", &len, &sum, &used) != 2 || used <= 0) return false;
+    const std::string name = head.substr((size_t)used);
+    if (name.empty() || name.find(' ') != std::string::npos) return false;
+    if (len == 0 || blob.size() - (nl + 1) != len) return false;
+    if (fnv1a(blob.data() + nl + 1, (size_t)len) != sum) return false;
+    lowered = name;
+    code = blob.substr(nl + 1);
+    return true;
+}
+
+void jit_cache_stats(unsigned long long out[4])
+{
+    JitStats& st = jit_stats();
+    out[0] = st.mem_hits;
+    out[1] = st.disk_hits;
+    out[2] = st.compiles;
+    out[3] = st.rejected;
+}
+
+namespace {
+// Kernel for a JIT translation unit: the in-memory table, then the disk cache
+// (keyed by source, arch and hipRTC version), then hipRTC.  With compile ==
+// false only the in-memory table is consulted (no compilation, no module load).
+bool jit_get(const std::string& src, const std::string& name_expr, const std::string& fn_name, KernelRef& out,
+             bool compile = true)
 {
     Context& c = context();
     std::string key_text = src + '\x01' + name_expr + '\x01' + c.arch + '\x01' + hiprtc_version();
-    for (int i = 0; i < k_embedded_count; i++) key_text += k_embedded_sources[i];
-    uint64_t key = fnv1a(key_text.data(), key_text.size());
+    uint64_t key = fnv1a(key_text.data(), key_text.size(), embedded_sources_hash());
     Registry& R = registry();
+    JitStats& stats = jit_stats();
     {
         std::lock_guard<std::mutex> lk(R.mtx);
         auto it = R.jit.find(key);
         if (it != R.jit.end()) {
             out = it->second;
+            stats.mem_hits++;
             return true;
         }
     }
+    if (!compile) return false;
     std::string code, lowered = fn_name;
     std::string dir = cache_dir();
     char hex[32];
@@ -490,29 +563,32 @@ bool jit_get(const std::string& src, const std::string& name_expr, const std::st
     if (!fn.empty()) {
         std::string blob;
         if (read_file(fn, blob)) {
-            size_t nl = blob.find('\n');
-            if (nl != std::string::npos && nl + 1 < blob.size()) {
-                lowered = blob.substr(0, nl);
-                code = blob.substr(nl + 1);
-                have = true;
+            have = parse_cache_entry(blob, lowered, code);
+            if (!have) {
+                lowered = fn_name;
+                stats.rejected++;
             }
         }
     }
-    if (!have) {
+    if (have) {
+        stats.disk_hits++;
+    } else {
         if (!rtc_compile(src, name_expr, c.arch, code, lowered, nullptr)) return false;
+        stats.compiles++;
         if (name_expr.empty()) lowered = fn_name;
         if (!fn.empty()) {
             mkdirs(dir + "/" + safe_arch);
-            write_file_atomic(fn, lowered + "\n" + code);
+            write_file_atomic(fn, make_cache_entry(lowered, code));
         }
     }
     hipModule_t mod;
     if (hipModuleLoadData(&mod, code.data()) != hipSuccess) {
         (void)hipGetLastError();
-        if (have) {  // corrupt cache entry: recompile once
+        if (have) {  // a valid entry the loader still refuses (e.g. another driver): recompile once
             if (!rtc_compile(src, name_expr, c.arch, code, lowered, nullptr)) return false;
+            stats.compiles++;
             if (name_expr.empty()) lowered = fn_name;
-            if (!fn.empty()) write_file_atomic(fn, lowered + "\n" + code);
+            if (!fn.empty()) write_file_atomic(fn, make_cache_entry(lowered, code));
             if (!check_hip(hipModuleLoadData(&mod, code.data()), "hipModuleLoadData")) return false;
         } else {
             TRTC_ERR("hipModuleLoadData failed");

@@ -122,6 +122,14 @@ bool get_source_kernel(const std::string& kernel_src, const std::string& fn_name
 bool compile_only(const std::string& arch, const std::string& name_expr, const std::string& type_text,
                   std::string* log);
 
+// Disk cache entries of run-time compiled code objects: the header is checked
+// (magic, code length, FNV-1a checksum) before anything is handed to the
+// loader.  True and the parts for a well-formed entry.
+bool parse_cache_entry(const std::string& blob, std::string& lowered, std::string& code);
+// JIT lookups so far: in-memory hits, disk-cache hits, hipRTC compilations,
+// disk entries rejected by parse_cache_entry.
+void jit_cache_stats(unsigned long long out[4]);
+
 bool launch_kernel(const KernelRef& k, dim3 grid, dim3 block, void** args, size_t shmem, hipStream_t s);
 bool sync_if_debug(hipStream_t s, const char* what);
 // Kernel timing: while enabled, every launch is bracketed by HIP events on its

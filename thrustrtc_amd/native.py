@@ -180,6 +180,8 @@ SIGNATURES = {
     "n_get_stream": (_vp, []),
     "n_dvvectorlike_data": (_vp, [_vp]),
     "n_jit_compile_check": (_i, [_s, _s]),
+    "n_jit_cache_stats": (None, [ctypes.POINTER(_ull)]),
+    "n_jit_cache_entry_valid": (_i, [ctypes.c_char_p, _ull]),
     "n_profile_begin": (_i, []),
     "n_profile_end": (_ull, [ctypes.c_char_p, _ull]),
 }
