@@ -271,6 +271,18 @@ TRTC_CAPI void n_jit_cache_stats(unsigned long long* out4);
  * length and checksum match), else 0; host only, needs no device.  The loader
  * sees only entries that pass this check. */
 TRTC_CAPI int n_jit_cache_entry_valid(const char* blob, unsigned long long size);
+/* No counterpart in the reference.  n_inclusive_scan (api.h:145) for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): same arguments, results
+ * and return codes, but the scan's lookback state lives in a 1 MiB
+ * per-GPU static device array, so the call allocates nothing (no hipMalloc,
+ * no pool, no module load).  Limits: n <= 65535 * 16384 for elements of up to
+ * 4 bytes, n <= 32767 * 16384 for 8-byte elements (larger inputs return -1
+ * before anything is launched); the kernel must be built in (Plus / Maximum /
+ * Minimum on plain vectors of built-in arithmetic types) or compiled by an
+ * earlier n_inclusive_scan of the same views and functor, else -1.  Calls
+ * that overlap on one GPU share the array and race: serialise them. */
+TRTC_CAPI int n_inclusive_scan_noalloc(void* ptr_vec_in, void* ptr_vec_out, void* ptr_binary_op);
 /* Kernel timing: between n_profile_begin and n_profile_end every kernel
  * launch is bracketed by HIP events on its stream; n_profile_end synchronises
  * and writes {"<kernel>": {"count": c, "total_ms": t}, ...} into json_out

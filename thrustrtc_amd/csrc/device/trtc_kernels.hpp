@@ -26,6 +26,18 @@ extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_fill_u32(unsigned*
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Zeroes n 32-bit words with relaxed agent-scope atomic stores: lookback
+// state cleared this way is seen by the next kernel's agent-scope atomics
+// however the memory is cached (used for the library's static scratch array,
+// which lives in a code object's global segment rather than in hipMalloc'd
+// memory).
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_clear_u32_agent(unsigned* p, size_t n)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        __hip_atomic_store(p + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Byte-wise gather for value types whose size is not 1/2/4/8:
 //   dst[i] = src[idx[i]], elem_size bytes each.
 extern "C" __global__ void __launch_bounds__(TRTC_BLOCK)

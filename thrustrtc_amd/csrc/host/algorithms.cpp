@@ -105,6 +105,24 @@ struct LookbackScratch
         incl = agg + tiles * W;
         return fill_u32(buf.p, bytes / 4, 0, s);
     }
+    // The same layout in caller-provided memory (nothing is allocated),
+    // cleared with agent-scope atomic stores: the memory is the library's
+    // static device array, and its clear must reach the scan's agent-scope
+    // atomics (tile counter, status words) whatever its caching.
+    bool use(void* mem, size_t capacity, size_t tiles, size_t value_bytes, hipStream_t s)
+    {
+        size_t W = words_for(value_bytes);
+        size_t bytes = 16 + 2 * tiles * W * 8;
+        if (bytes > capacity) {
+            TRTC_ERR("scan: %zu tiles need %zu bytes of lookback scratch, more than the %zu available", tiles,
+                     bytes, capacity);
+            return false;
+        }
+        counter = (unsigned*)mem;
+        agg = (unsigned long long*)((char*)mem + 16);
+        incl = agg + tiles * W;
+        return clear_u32_agent(mem, bytes / 4, s);
+    }
 };
 
 bool radix_elem(const std::string& elem, int& key_bytes, int& kind) { return radix_key_info(elem, key_bytes, kind); }
@@ -377,8 +395,11 @@ bool sort_dispatch(DVVectorLike& keys, DVVectorLike* values, const Functor& comp
 
 // ---------------------------------------------------------------- scan
 
+// With fixed_scratch, the lookback state lives in that caller memory and the
+// kernel must already exist (ahead-of-time, or compiled by an earlier call):
+// the call then allocates nothing and compiles nothing.
 bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op, const DeviceViewable* init,
-                  bool exclusive)
+                  bool exclusive, void* fixed_scratch = nullptr, size_t fixed_bytes = 0)
 {
     if (!ensure_init()) return false;
     size_t n = vin.size();
@@ -393,13 +414,21 @@ bool general_scan(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op
     if (!init_value(init, vin.name_elem_cls(), vout.name_elem_cls(), initv)) return false;
     std::string name = "trtc_scan_k<" + vcls(vin) + ", " + vcls(vout) + ", " + vcls(op) + ", " + T + ">";
     KernelRef k;
-    if (!get_template_kernel(name, name, k)) return false;
+    if (fixed_scratch) {
+        if (!find_template_kernel(name, name, k)) {
+            TRTC_ERR("scan without allocation: %s is neither built in nor compiled by an earlier scan call of "
+                     "this process", name.c_str());
+            return false;
+        }
+    } else if (!get_template_kernel(name, name, k)) {
+        return false;
+    }
     hipStream_t s = cur_stream();
     const unsigned block = tsz == 8 ? 1024u : kScanBlock;  // ScanBlock<T> in the device header
     size_t tile = (size_t)block * tile_items_for(tsz);
     size_t tiles = (n + tile - 1) / tile;
     LookbackScratch lb;
-    if (!lb.make(tiles, tsz, s)) return false;
+    if (fixed_scratch ? !lb.use(fixed_scratch, fixed_bytes, tiles, tsz, s) : !lb.make(tiles, tsz, s)) return false;
     trtc::ScanArgs a{n, {lb.agg, lb.incl, lb.counter}, exclusive ? 1 : 0};
     Args args;
     args.view(vin);
@@ -728,6 +757,13 @@ bool reduce_view(const DVVectorLike& vec, const DeviceViewable* init, const Func
 bool extreme_view(const DVVectorLike& vec, const Functor& comp, bool max, size_t& id, bool& handled)
 {
     return general_extreme(vec, comp, max, id, handled);
+}
+
+bool inclusive_scan_fixed_scratch(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op, void* scratch,
+                                  size_t scratch_bytes)
+{
+    if (!scratch) return false;
+    return general_scan(vin, vout, op, nullptr, false, scratch, scratch_bytes);
 }
 
 bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,

@@ -658,6 +658,21 @@ bool get_template_kernel(const std::string& name_expr, const std::string& type_t
     return true;
 }
 
+bool find_template_kernel(const std::string& name_expr, const std::string& type_text, KernelRef& out)
+{
+    const void* a = find_aot(name_expr);
+    if (a) {
+        out.aot = a;
+        out.jit = nullptr;
+        out.name = stable_name(name_expr);
+        return true;
+    }
+    std::string src = assemble_source("", type_text, true);
+    if (!jit_get(src, name_expr, "", out, false)) return false;
+    out.name = stable_name(name_expr);
+    return true;
+}
+
 bool get_source_kernel(const std::string& kernel_src, const std::string& fn_name, KernelRef& out)
 {
     std::string src = assemble_source(kernel_src, kernel_src, false);
@@ -884,6 +899,15 @@ bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s)
     KernelRef k;
     if (!get_builtin("trtc_fill_u32", k)) return false;
     void* args[] = {&p, &words, &v};
+    return launch_kernel(k, dim3(grid_for(words, 256)), dim3(256), args, 0, s);
+}
+
+bool clear_u32_agent(void* p, size_t words, hipStream_t s)
+{
+    if (!words) return true;
+    KernelRef k;
+    if (!get_builtin("trtc_clear_u32_agent", k)) return false;
+    void* args[] = {&p, &words};
     return launch_kernel(k, dim3(grid_for(words, 256)), dim3(256), args, 0, s);
 }
 

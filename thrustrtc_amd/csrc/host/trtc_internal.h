@@ -116,6 +116,9 @@ const void* find_aot(const std::string& name_expr);
 // it exists, otherwise compiled by hipRTC (cached in memory and on disk).
 // `type_text` names every type involved (used to pull in struct definitions).
 bool get_template_kernel(const std::string& name_expr, const std::string& type_text, KernelRef& out);
+// The same lookup without compiling or loading anything: an ahead-of-time
+// kernel, or one an earlier call already compiled and loaded in this process.
+bool find_template_kernel(const std::string& name_expr, const std::string& type_text, KernelRef& out);
 // An extern "C" kernel named `fn_name` defined in `kernel_src` (user kernels).
 bool get_source_kernel(const std::string& kernel_src, const std::string& fn_name, KernelRef& out);
 // Compile only (no device needed): for offline cache warm-up and checks.
@@ -151,6 +154,9 @@ void pinned_release(void* p, hipStream_t s);
 
 // Built-in utility kernels.
 bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s);
+// Zeroes words with agent-scope atomic stores (lookback state in static
+// device memory, read by the next kernel's agent-scope atomics).
+bool clear_u32_agent(void* p, size_t words, hipStream_t s);
 // Device-to-device copy of nbytes: a 16-byte-per-thread kernel when both
 // pointers are 16-byte aligned, hipMemcpyAsync otherwise.
 bool copy_bytes(const void* src, void* dst, size_t nbytes, hipStream_t s);
@@ -160,6 +166,12 @@ bool gather_bytes(const void* src, void* dst, const unsigned* idx, size_t n, siz
 // Radix key class of an element type: width in bytes and kind (0 unsigned /
 // bool, 1 signed, 2 floating point); false for other types.
 bool radix_key_info(const std::string& elem, int& key_bytes, int& kind);
+
+// Inclusive_Scan with its lookback state in caller memory of scratch_bytes
+// (allocates and compiles nothing; fails if the kernel does not exist yet or
+// the input needs more scratch).
+bool inclusive_scan_fixed_scratch(const DVVectorLike& vin, DVVectorLike& vout, const Functor& op, void* scratch,
+                                  size_t scratch_bytes);
 
 // Single-call onesweep radix sort (the TRTC_Sort_By_Key path) reading its
 // input from keys_src/vals_src and leaving the result in keys/vals.

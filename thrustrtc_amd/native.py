@@ -180,6 +180,7 @@ SIGNATURES = {
     "n_get_stream": (_vp, []),
     "n_dvvectorlike_data": (_vp, [_vp]),
     "n_jit_compile_check": (_i, [_s, _s]),
+    "n_inclusive_scan_noalloc": (_i, [_vp, _vp, _vp]),
     "n_jit_cache_stats": (None, [ctypes.POINTER(_ull)]),
     "n_jit_cache_entry_valid": (_i, [ctypes.c_char_p, _ull]),
     "n_profile_begin": (_i, []),
