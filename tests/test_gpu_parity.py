@@ -213,6 +213,56 @@ def test_sort_all_equal_and_sorted(trtc):
         assert (dk.to_host() == arr[perm]).all() and (dv.to_host() == perm).all()
 
 
+def constant_digit_keys(wl, case, n, seed):
+    """Keys some of whose 8-bit digits are the same in every key (the passes
+    over those digits are skipped; reference radix_sort.cpp:323-325, 351-353)."""
+    x = wl.u32_uniform(n, seed)
+    if case == "below_2^16":        # digits 2, 3 constant: two passes run
+        return x & np.uint32(0xFFFF), "uint32_t"
+    if case == "middle_only":       # digits 0 and 3 constant
+        return (x & np.uint32(0x00FFFF00)) | np.uint32(0x5A000077), "uint32_t"
+    if case == "one_digit":         # digits 0, 2, 3 constant: one pass runs (odd: result copied back)
+        return (x & np.uint32(0xFF00)) | np.uint32(0x13000000), "uint32_t"
+    if case == "three_digits":      # digit 1 constant: three passes run
+        return (x & np.uint32(0xFFFF00FF)) | np.uint32(0x4200), "uint32_t"
+    if case == "all_equal":         # no pass runs
+        return np.full(n, 0xDEADBEEF, dtype=np.uint32), "uint32_t"
+    if case == "i32_small":         # signed keys in [-100, 100]: top digits 0x00 / 0xFF vary, so all run
+        return wl.i32_range(n, seed, -100, 100), "int32_t"
+    if case == "f32_const_sign":    # floats in [1, 2): sign/exponent byte constant
+        return (np.float32(1.0) + (x >> np.uint32(9)).astype(np.float32) * np.float32(2.0 ** -23)), "float"
+    if case == "u64_below_2^40":    # 8-byte keys, digits 5..7 constant
+        return (x.astype(np.uint64) << np.uint64(8)) | np.uint64(0x11), "uint64_t"
+    raise ValueError(case)
+
+
+CONST_CASES = ["below_2^16", "middle_only", "one_digit", "three_digits", "all_equal", "i32_small",
+               "f32_const_sign", "u64_below_2^40"]
+
+
+@pytest.mark.parametrize("case", CONST_CASES)
+@pytest.mark.parametrize("n", [100003, (1 << 20) + 3, 1 << 22])
+@pytest.mark.parametrize("greater", [False, True])
+def test_sort_by_key_constant_digits(trtc, orc, wl, case, n, greater):
+    say(f"constant digits {case} n={n} greater={greater}")
+    keys, elem = constant_digit_keys(wl, case, n, 13)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv, trtc.Greater() if greater else None)
+    perm = orc.sort_perm(keys, elem, greater)
+    assert np.array_equal(dk.to_host().view(f"u{keys.itemsize}"), keys[perm].view(f"u{keys.itemsize}"))
+    assert np.array_equal(dv.to_host(), perm.astype(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["below_2^16", "one_digit", "all_equal"])
+def test_sort_keys_only_constant_digits(trtc, orc, wl, case):
+    say(f"keys-only constant digits {case}")
+    keys, elem = constant_digit_keys(wl, case, (1 << 21) + 5, 17)
+    dk = trtc.device_vector_from_numpy(keys)
+    trtc.Sort(dk)
+    assert np.array_equal(dk.to_host(), orc.sort(keys, elem))
+
+
 # ---------------------------------------------------------------- scan
 SCAN_SIZES = [1, 2, 1000, 4095, 4096, 4097, 100003, 1 << 20]
 

@@ -47,6 +47,18 @@ struct ScanByKeyArgs
     int exclusive;
 };
 
+// Device-side plan of one single-sequence radix sort, written by the
+// histogram pre-pass once it knows which digits are constant (one bin holds
+// every key): per pass the key / value buffers to read and write (kin 0: the
+// pass is skipped), and the copy that brings the result into the caller's
+// buffers when an odd number of passes ran in place (ksrc 0: none).  The
+// passes and the copy read it, so skipping needs no host synchronisation.
+struct RadixPlan
+{
+    unsigned long long kin[8], kout[8], vin[8], vout[8];
+    unsigned long long copy_ksrc, copy_kdst, copy_vsrc, copy_vdst;
+};
+
 struct RadixHistArgs
 {
     const void* keys;
@@ -59,6 +71,14 @@ struct RadixHistArgs
     unsigned* done;                   // workgroup completion counter, zero on entry
     unsigned* zero_ptr;               // words to clear for the passes (status + tile counters)
     size_t zero_words;
+    // buffers of the sort for the plan (plan null: no plan is written)
+    RadixPlan* plan;
+    const void* vals_src;             // first pass input (keys: `keys`)
+    void* keys_dst;                   // caller's buffers (result)
+    void* vals_dst;
+    void* keys_tmp;                   // ping-pong buffers
+    void* vals_tmp;
+    int val_bytes;
 };
 
 struct RadixPassArgs
@@ -74,6 +94,15 @@ struct RadixPassArgs
     int shift;
     int kind;
     int descending;
+    const RadixPlan* plan;                  // non-null: buffers from plan (pass `pass`), skip when kin is 0
+    int pass;
+};
+
+struct RadixCopyArgs
+{
+    const RadixPlan* plan;
+    size_t key_bytes;                       // n * key width
+    size_t val_bytes;                       // n * value width (0: keys only)
 };
 
 struct MergeArgs

@@ -38,6 +38,12 @@ extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_clear_u32_agent(un
         __hip_atomic_store(p + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Copy-back of a single radix sort's plan (nothing when the plan needs none).
+extern "C" __global__ void __launch_bounds__(TRTC_BLOCK) trtc_radix_copyback_k(trtc::RadixCopyArgs a)
+{
+    trtc::radix_copyback(a);
+}
+
 // Byte-wise gather for value types whose size is not 1/2/4/8:
 //   dst[i] = src[idx[i]], elem_size bytes each.
 extern "C" __global__ void __launch_bounds__(TRTC_BLOCK)

@@ -97,6 +97,46 @@ TRTC_DEV void st_relaxed(unsigned long long* p, unsigned long long v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The passes that run read and write alternately the ping-pong and the
+// caller's buffers so that, where possible, the last one writes the caller's
+// buffers: out of place (source != result) the first pass reads the source
+// and the parity is chosen backwards from the last pass; in place, the first
+// pass must leave the source, so an odd number of passes ends in the
+// ping-pong buffers and a copy brings the result back.  No pass runs when
+// every digit is constant (then only an out-of-place sort copies).
+template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigned constant_mask)
+{
+    RadixPlan& P = *a.plan;
+    const unsigned long long ks = (unsigned long long)a.keys, vs = (unsigned long long)a.vals_src;
+    const unsigned long long kd = (unsigned long long)a.keys_dst, vd = (unsigned long long)a.vals_dst;
+    const unsigned long long kt = (unsigned long long)a.keys_tmp, vt = (unsigned long long)a.vals_tmp;
+    const bool in_place = ks == kd;
+    int runs = 0;
+    for (int p = 0; p < KB; p++) runs += (constant_mask >> p) & 1u ? 0 : 1;
+    unsigned long long ck = ks, cv = vs;  // current location of the data
+    int i = 0;
+    for (int p = 0; p < 8; p++) {
+        P.kin[p] = P.kout[p] = P.vin[p] = P.vout[p] = 0;
+        if (p >= KB || ((constant_mask >> p) & 1u)) continue;
+        // pass number i of `runs`: write the result buffers when the passes
+        // left after this one are even in number (out of place), or
+        // alternate starting with the ping-pong buffers (in place)
+        const bool to_dst = in_place ? (i & 1) != 0 : ((runs - 1 - i) & 1) == 0;
+        P.kin[p] = ck;
+        P.vin[p] = cv;
+        P.kout[p] = to_dst ? kd : kt;
+        P.vout[p] = to_dst ? vd : vt;
+        ck = P.kout[p];
+        cv = P.vout[p];
+        i++;
+    }
+    const bool need_copy = ck != kd;
+    P.copy_ksrc = need_copy ? ck : 0;
+    P.copy_kdst = need_copy ? kd : 0;
+    P.copy_vsrc = need_copy && a.val_bytes ? cv : 0;
+    P.copy_vdst = need_copy && a.val_bytes ? vd : 0;
+}
+
 // One read of the keys for all digit histograms (16-byte nontemporal loads).
 // Also clears the passes' lookback state (grid-stride), so no separate
 // clearing launch is needed.  The last workgroup to finish turns the global
@@ -200,19 +240,26 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     }
     __syncthreads();
     if (!s_last) return;
-    // Last workgroup: exclusive scan of each 256-bin histogram.
+    // Last workgroup: exclusive scan of each 256-bin histogram; a pass whose
+    // digit has one bin holding all n keys is constant (reference
+    // radix_sort.cpp:323-325, 351-353: all-equal keys and constant bits).
     if (threadIdx.x < 64) {
+        unsigned constant_mask = 0;
         for (int p = 0; p < KB; p++) {
             unsigned long long run = 0;
+            bool whole = false;
             for (int c = 0; c < 4; c++) {
                 int d = c * 64 + (int)threadIdx.x;
                 unsigned long long v = ld_relaxed(a.hist + p * 256 + d);
+                whole = whole || v == (unsigned long long)a.n;
                 Plus plus;
                 unsigned long long inc = wave_inclusive_scan(v, plus);
                 a.digit_start[p * 256 + d] = run + inc - v;
                 run += shfl(inc, 63);
             }
+            if (ballot(whole)) constant_mask |= 1u << p;
         }
+        if (a.plan && threadIdx.x == 0) write_radix_plan<KB>(a, constant_mask);
     }
 }
 
@@ -490,18 +537,31 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
 }
 
 
-// One digit pass over the whole input.
+// One digit pass over the whole input.  With a plan, the buffers come from
+// it and a pass over a constant digit returns at once.
 template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND = -1>
 TRTC_DEV void radix_pass(const RadixPassArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
     typedef typename ValWord<VB>::U V;
+    const void* kin = a.keys_in;
+    const void* vin = a.vals_in;
+    void* kout = a.keys_out;
+    void* vout = a.vals_out;
+    if (a.plan) {
+        const RadixPlan& P = *a.plan;
+        if (!P.kin[a.pass]) return;
+        kin = (const void*)P.kin[a.pass];
+        vin = (const void*)P.vin[a.pass];
+        kout = (void*)P.kout[a.pass];
+        vout = (void*)P.vout[a.pass];
+    }
     auto make = [&](unsigned t) {
         RadixJob<U, V, S> j;
-        j.kin = (const U*)a.keys_in;
-        j.vin = (const V*)a.vals_in;
-        j.kout = (U*)a.keys_out;
-        j.vout = (V*)a.vals_out;
+        j.kin = (const U*)kin;
+        j.vin = (const V*)vin;
+        j.kout = (U*)kout;
+        j.vout = (V*)vout;
         j.n = a.n;
         j.tile = t;
         j.status = (S*)a.status;
@@ -510,6 +570,39 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
         return j;
     };
     radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
+}
+
+// The copy a plan may ask for after the passes (the result of an odd number
+// of in-place passes lies in the ping-pong buffers): grid-stride, 16 bytes
+// per step when both buffers and the length allow, else 4 or 1.  Returns at
+// once when the plan needs no copy.
+TRTC_DEV void radix_copy_span(const void* src, void* dst, size_t bytes)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t al = (size_t)src | (size_t)dst | bytes;
+    if ((al & 15) == 0) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const v4u* s = (const v4u*)src;
+        v4u* d = (v4u*)dst;
+        for (size_t i = gid; i < bytes / 16; i += stride) d[i] = s[i];
+    } else if ((al & 3) == 0) {
+        const unsigned* s = (const unsigned*)src;
+        unsigned* d = (unsigned*)dst;
+        for (size_t i = gid; i < bytes / 4; i += stride) d[i] = s[i];
+    } else {
+        const unsigned char* s = (const unsigned char*)src;
+        unsigned char* d = (unsigned char*)dst;
+        for (size_t i = gid; i < bytes; i += stride) d[i] = s[i];
+    }
+}
+
+TRTC_DEV void radix_copyback(const RadixCopyArgs& a)
+{
+    const RadixPlan& P = *a.plan;
+    if (!P.copy_ksrc) return;
+    radix_copy_span((const void*)P.copy_ksrc, (void*)P.copy_kdst, a.key_bytes);
+    if (a.val_bytes && P.copy_vsrc) radix_copy_span((const void*)P.copy_vsrc, (void*)P.copy_vdst, a.val_bytes);
 }
 
 // One digit pass over a batch of segments.  Pass 0 reads the segment's

@@ -161,14 +161,16 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
 
     PoolBuf ktmp(n * KB, s);
     PoolBuf vtmp(val_bytes ? n * val_bytes : 0, s);
-    // scratch: hist [KB*256 u64] | done u32 (+pad) | tile counters [8 u32] | status [KB][tiles][256] | digit_start
+    // scratch: hist [KB*256 u64] | done u32 (+pad) | tile counters [8 u32] | status [KB][tiles][256] |
+    // digit_start [KB*256 u64] | plan (the passes' buffers, written by the histogram pre-pass)
     size_t off_hist = 0;
     size_t off_done = off_hist + KB * 256 * 8;
     size_t off_cnt = off_done + 16;
     size_t off_status = align_up(off_cnt + 8 * 4, 256);
     size_t status_bytes_per_pass = align_up(tiles * 256 * sw, 256);
     size_t off_dstart = off_status + KB * status_bytes_per_pass;
-    size_t scratch_bytes = off_dstart + KB * 256 * 8;
+    size_t off_plan = align_up(off_dstart + KB * 256 * 8, 256);
+    size_t scratch_bytes = off_plan + align_up(sizeof(trtc::RadixPlan), 256);
     PoolBuf scratch(scratch_bytes, s);
     if (!ktmp.p || (val_bytes && !vtmp.p) || !scratch.p) return false;
     char* sc = (char*)scratch.p;
@@ -189,6 +191,16 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     ha.done = (unsigned*)(sc + off_done);
     ha.zero_ptr = (unsigned*)(sc + off_cnt);
     ha.zero_words = (off_dstart - off_cnt) / 4;
+    // passes over constant digits (one bin holds all n keys) are skipped
+    // through the device-side plan, without a host synchronisation
+    trtc::RadixPlan* plan = (trtc::RadixPlan*)(sc + off_plan);
+    ha.plan = plan;
+    ha.vals_src = vals_src;
+    ha.keys_dst = keys;
+    ha.vals_dst = vals;
+    ha.keys_tmp = ktmp.p;
+    ha.vals_tmp = vtmp.p;
+    ha.val_bytes = val_bytes;
     // 1024-thread workgroups (TRTC_HIST_BLOCK), at most one per CU for their
     // LDS, at least 16 keys per thread: every workgroup adds its KB*256 bins
     // into the same global counters, so a grid larger than the work needs
@@ -222,15 +234,22 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         pa.shift = 8 * p;
         pa.kind = kind;
         pa.descending = descending ? 1 : 0;
+        pa.plan = plan;
+        pa.pass = p;
         void* args[] = {&pa};
         if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kRadixBlock), args, 0, s)) return false;
     }
-    if (KB & 1) {
-        if (!copy_bytes(ktmp.p, keys, n * KB, s))
-            return false;
-        if (val_bytes &&
-            !copy_bytes(vtmp.p, vals, n * val_bytes, s))
-            return false;
+    // the copy the plan may need (an odd number of in-place passes ran, or
+    // none ran out of place); its workgroups return at once otherwise
+    KernelRef ck;
+    if (!get_template_kernel("trtc_radix_copyback_k", "", ck)) return false;
+    trtc::RadixCopyArgs ca{plan, n * (size_t)KB, n * (size_t)val_bytes};
+    {
+        void* args[] = {&ca};
+        const size_t vecs = (n * (size_t)KB + 15) / 16;
+        const unsigned cgrid = (unsigned)std::min<size_t>((vecs + 255) / 256,
+                                                          (size_t)std::max(1, context().num_cus) * 8);
+        if (!launch_kernel(ck, dim3(std::max(1u, cgrid)), dim3(256), args, 0, s)) return false;
     }
     return true;
 }
