@@ -1,15 +1,9 @@
-// Batched-segment kernels: radix passes and histograms for the tile shapes
-// the batch autotuner chooses among, segmented scans and reductions for the
-// built-in element types.
+// Batched-segment kernels: digit histograms, segmented scans and reductions
+// for the built-in element types (the digit passes are in aot_seg_pass_k*.hip,
+// one file per key kind so that they compile in parallel).
 #include "aot_common.h"
 
 namespace {
-template <int KB, int VB, int BLOCK, int ITEMS> void reg_seg_pass()
-{
-    std::string p = std::to_string(KB) + ", " + std::to_string(VB) + ", " + std::to_string(BLOCK) + ", " +
-                    std::to_string(ITEMS) + ">";
-    TRTC_REG(("trtc_seg_radix_pass_k<" + p).c_str(), (&trtc_seg_radix_pass_k<KB, VB, BLOCK, ITEMS>));
-}
 template <int KB> void reg_seg_hist()
 {
     TRTC_REG(("trtc_seg_radix_hist_k<" + std::to_string(KB) + ">").c_str(), (&trtc_seg_radix_hist_k<KB>));
@@ -31,23 +25,7 @@ struct RegSeg
 {
     RegSeg()
     {
-        // 4-byte keys: keys only and 4-byte values
-        reg_seg_pass<4, 4, 1024, 16>();
-        reg_seg_pass<4, 4, 512, 16>();
-        reg_seg_pass<4, 4, 512, 8>();
-        reg_seg_pass<4, 4, 256, 16>();
-        reg_seg_pass<4, 4, 256, 8>();
-        reg_seg_pass<4, 0, 1024, 16>();
-        reg_seg_pass<4, 0, 512, 16>();
-        reg_seg_pass<4, 0, 256, 8>();
         reg_seg_hist<4>();
-        // 8-byte keys with 8-byte or no values
-        reg_seg_pass<8, 8, 1024, 8>();
-        reg_seg_pass<8, 8, 512, 8>();
-        reg_seg_pass<8, 8, 256, 8>();
-        reg_seg_pass<8, 0, 1024, 8>();
-        reg_seg_pass<8, 0, 512, 8>();
-        reg_seg_pass<8, 0, 256, 8>();
         reg_seg_hist<8>();
         for (const char* op : {"Plus", "Maximum", "Minimum"}) {
             reg_seg_scan<int32_t, 16>("int32_t", op);

@@ -220,11 +220,12 @@ __global__ void __launch_bounds__(BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a
 
 // ---- batched segments (one launch over many independent sequences) ----
 // Tile shapes are template arguments so the batch autotuner can pick among
-// instantiations; every one keeps one thread per digit (BLOCK >= 256).
-template <int KB, int VB, int BLOCK, int ITEMS>
+// instantiations; every one keeps one thread per digit (BLOCK >= 256).  KIND
+// fixes the key kind at compile time (trtc::KeyKind; -1 reads it).
+template <int KB, int VB, int BLOCK, int ITEMS, int KIND>
 __global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArgs a)
 {
-    trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB>(a);
+    trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB, KIND>(a);
 }
 
 template <int KB>

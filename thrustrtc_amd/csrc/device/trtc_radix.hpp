@@ -611,7 +611,7 @@ TRTC_DEV void radix_copyback(const RadixCopyArgs& a)
 // ends there.  A one-tile segment has no lookback and
 // no pre-pass histogram, and sorts in place (its tile holds every key in
 // registers before it writes).
-template <int BLOCK, int ITEMS, int KB, int VB>
+template <int BLOCK, int ITEMS, int KB, int VB, int KIND = -1>
 TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
@@ -637,7 +637,7 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         }
         return j;
     };
-    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned, -1>(a.tile_counter, make, a.shift, a.kind, a.descending);
+    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per

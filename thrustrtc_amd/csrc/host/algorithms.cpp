@@ -240,15 +240,15 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kRadixBlock), args, 0, s)) return false;
     }
     // the copy the plan may need (an odd number of in-place passes ran, or
-    // none ran out of place); its workgroups return at once otherwise
+    // none ran out of place); its workgroups return at once otherwise, so the
+    // grid is at most one workgroup per CU (grid-stride when it copies)
     KernelRef ck;
     if (!get_template_kernel("trtc_radix_copyback_k", "", ck)) return false;
     trtc::RadixCopyArgs ca{plan, n * (size_t)KB, n * (size_t)val_bytes};
     {
         void* args[] = {&ca};
         const size_t vecs = (n * (size_t)KB + 15) / 16;
-        const unsigned cgrid = (unsigned)std::min<size_t>((vecs + 255) / 256,
-                                                          (size_t)std::max(1, context().num_cus) * 8);
+        const unsigned cgrid = (unsigned)std::min<size_t>((vecs + 255) / 256, (size_t)std::max(1, context().num_cus));
         if (!launch_kernel(ck, dim3(std::max(1u, cgrid)), dim3(256), args, 0, s)) return false;
     }
     return true;

@@ -62,6 +62,16 @@ SortSchedule default_sort_schedule(int key_bytes, int val_bytes)
     return {{kNoLimit, 512, 16}};
 }
 
+// Size classes for batches whose shape changes from call to call (the request
+// server): small segments in small tiles, segments above 2^18 keys in the
+// 1024-thread tiles of large single sorts (the autotuner's choice on
+// log-uniform sizes, e.g. the 4096-segment batch).
+SortSchedule served_sort_schedule(int key_bytes, int val_bytes)
+{
+    if (key_bytes > 4 || val_bytes > 4) return {{2048, 256, 8}, {1u << 18, 512, 8}, {kNoLimit, 1024, 8}};
+    return {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}};
+}
+
 bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool descending,
                const SortSchedule& sched, SortPlan& plan)
 {
@@ -206,7 +216,7 @@ std::string seg_hist_name(const SortPlan& plan) { return "trtc_seg_radix_hist_k<
 std::string seg_pass_name(const SortPlan& plan, const SortPlan::Cls& C)
 {
     return "trtc_seg_radix_pass_k<" + std::to_string(plan.KB) + ", " + std::to_string(plan.VB) + ", " +
-           std::to_string(C.block) + ", " + std::to_string(C.items) + ">";
+           std::to_string(C.block) + ", " + std::to_string(C.items) + ", " + std::to_string(plan.kind) + ">";
 }
 std::string seg_scan_name(const ScanPlan& plan)
 {

@@ -292,7 +292,7 @@ public:
                 }
                 int vb = f.op == TRTC_REQ_SORT_BY_KEY ? f.VB : 0;
                 SortPlan plan;
-                if (!plan_sort(segs, f.KB, vb, f.kind, false, default_sort_schedule(f.KB, vb), plan)) return false;
+                if (!plan_sort(segs, f.KB, vb, f.kind, false, served_sort_schedule(f.KB, vb), plan)) return false;
                 char* sc = (char*)get_mem(plan.scratch_bytes, s);
                 char* tk = plan.tmp_elems ? (char*)get_mem(plan.tmp_elems * f.KB, s) : nullptr;
                 char* tv = plan.tmp_elems && vb ? (char*)get_mem(plan.tmp_elems * vb, s) : nullptr;

@@ -30,6 +30,7 @@ struct SortClass
 typedef std::vector<SortClass> SortSchedule;
 std::string describe(const SortSchedule& s);
 SortSchedule default_sort_schedule(int key_bytes, int val_bytes);
+SortSchedule served_sort_schedule(int key_bytes, int val_bytes);
 
 struct SortPlan
 {
