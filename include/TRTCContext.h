@@ -74,6 +74,6 @@ private:
 // Stream used by this thread's calls (default: the null stream, like the
 // reference).  `stream` is a hipStream_t.
 void THRUST_RTC_API TRTC_Set_Stream(void* stream);
-void* THRUST_RTC_API TRTC_Get_Stream();
+THRUST_RTC_API void* TRTC_Get_Stream();
 
 #endif

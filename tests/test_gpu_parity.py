@@ -537,3 +537,51 @@ def test_sort_user_comparator_known_answer(trtc):
     d = trtc.device_vector_from_list([1, 4, 2, 8, 5, 7], "int32_t")
     trtc.Sort(d, trtc.Functor({}, ["x", "y"], "        return x > y;\n"))
     assert d.to_host().tolist() == [8, 7, 5, 4, 2, 1]
+
+
+# ---------------------------------------------------------------- For
+def test_for_per_index_semantics(trtc):
+    """The For kernel takes several indices per thread, but the body keeps
+    the semantics of one index per thread (impl_context.inl:851-898): a
+    `return` ends that index only, a write to a parameter does not carry over
+    to the next index, begin / end bound every index."""
+    n = 100003
+    out = trtc.device_vector_from_numpy(np.full(n + 10, -1, dtype=np.int64))
+    loop = trtc.For(["out", "k"], "i", """
+    if (i % 3 == 0) return;
+    k = k + (long long)i;
+    out[i] = k;
+""")
+    loop.launch(5, n, [out, trtc.DVInt64(1000)])
+    got = out.to_host()
+    idx = np.arange(n + 10)
+    exp = np.where((idx >= 5) & (idx < n) & (idx % 3 != 0), 1000 + idx, -1)
+    assert np.array_equal(got, exp)
+    # the same For again (the resolved kernel is reused) and after a code block
+    loop.launch(0, 7, [out, trtc.DVInt64(0)])
+    assert out.to_host()[:7].tolist() == [-1, 1, 2, -1, 4, 5, -1]
+    trtc.Add_Code_Block("__device__ inline long long trtc_test_twice(long long x) { return 2 * x; }\n")
+    loop2 = trtc.For(["out"], "i", "    out[i] = trtc_test_twice((long long)i);\n")
+    loop2.launch_n(9, [out])
+    assert out.to_host()[:9].tolist() == [2 * i for i in range(9)]
+
+
+def test_replace_copy_reads_each_element_once(trtc):
+    """Replace_Copy / Replace_Copy_If evaluate a computed input once per
+    element (replace.cpp:27-28, 39-40): a custom vector that counts its
+    evaluations is read n times."""
+    n = 70001
+    cnt = trtc.device_vector_from_numpy(np.zeros(1, dtype=np.uint64))
+    src = trtc.DVCustomVector({"cnt": cnt}, "idx", """
+        atomicAdd((unsigned long long*)&cnt[0], 1ull);
+        return (int)(idx % 7);
+""", "int32_t", n)
+    out = trtc.device_vector("int32_t", n)
+    trtc.Replace_Copy(src, out, trtc.DVInt32(3), trtc.DVInt32(-1))
+    exp = np.arange(n) % 7
+    assert np.array_equal(out.to_host(), np.where(exp == 3, -1, exp))
+    assert int(cnt.to_host()[0]) == n
+    odd = trtc.Functor({}, ["x"], "        return x % 2 == 1;\n")
+    trtc.Replace_Copy_If(src, out, odd, trtc.DVInt32(100))
+    assert np.array_equal(out.to_host(), np.where(exp % 2 == 1, 100, exp))
+    assert int(cnt.to_host()[0]) == 2 * n

@@ -117,27 +117,35 @@ bool run_map(const std::vector<const char*>& names, const char* value, const cha
     std::vector<const DeviceViewable*> a(args);
     std::string sig;
     for (size_t i = 0; i < names.size(); i++) sig += std::string(", ") + a[i]->name_view_cls() + " " + names[i];
-    std::string src = std::string("extern \"C\" __global__ void __launch_bounds__(256) trtc_map(size_t _trtc_n") + sig +
-                      ")\n{\n"
-                      "    const size_t _trtc_b = (size_t)blockIdx.x * 2048 + threadIdx.x;\n"
-                      "    auto _trtc_val = [&](size_t idx) { return " + value + "; };\n"
-                      "    typedef decltype(_trtc_val((size_t)0)) _trtc_T;\n"
-                      "    auto _trtc_put = [&](size_t idx, const _trtc_T& _x) { " + store + " };\n"
-                      "    if (_trtc_b + 7 * 256 < _trtc_n) {\n"
-                      "        _trtc_T _r[8];\n"
-                      "#pragma unroll\n"
-                      "        for (int _k = 0; _k < 8; _k++) _r[_k] = _trtc_val(_trtc_b + (size_t)_k * 256);\n"
-                      "#pragma unroll\n"
-                      "        for (int _k = 0; _k < 8; _k++) _trtc_put(_trtc_b + (size_t)_k * 256, _r[_k]);\n"
-                      "    } else {\n"
-                      "        for (int _k = 0; _k < 8; _k++) {\n"
-                      "            const size_t idx = _trtc_b + (size_t)_k * 256;\n"
-                      "            if (idx < _trtc_n) _trtc_put(idx, _trtc_val(idx));\n"
-                      "        }\n"
-                      "    }\n"
-                      "}\n";
     KernelRef k;
-    if (!get_source_kernel(src, "trtc_map", k)) return false;
+    const std::string memo = std::string("map\x01") + sig + "\x01" + value + "\x01" + store;
+    auto build = [&](KernelRef& out) {
+        std::string src = std::string("extern \"C\" __global__ void __launch_bounds__(256) trtc_map(size_t _trtc_n") +
+                          sig +
+                          ")\n{\n"
+                          "    auto _trtc_val = [&](size_t idx) { return " + value + "; };\n"
+                          "    typedef decltype(_trtc_val((size_t)0)) _trtc_T;\n"
+                          "    auto _trtc_put = [&](size_t idx, const _trtc_T& _x) { " + store + " };\n"
+                          "    const size_t _trtc_stride = (size_t)gridDim.x * 2048;\n"
+                          "    for (size_t _trtc_b = (size_t)blockIdx.x * 2048 + threadIdx.x; _trtc_b < _trtc_n;\n"
+                          "         _trtc_b += _trtc_stride) {\n"
+                          "        if (_trtc_b + 7 * 256 < _trtc_n) {\n"
+                          "            _trtc_T _r[8];\n"
+                          "#pragma unroll\n"
+                          "            for (int _k = 0; _k < 8; _k++) _r[_k] = _trtc_val(_trtc_b + (size_t)_k * 256);\n"
+                          "#pragma unroll\n"
+                          "            for (int _k = 0; _k < 8; _k++) _trtc_put(_trtc_b + (size_t)_k * 256, _r[_k]);\n"
+                          "        } else {\n"
+                          "            for (int _k = 0; _k < 8; _k++) {\n"
+                          "                const size_t idx = _trtc_b + (size_t)_k * 256;\n"
+                          "                if (idx < _trtc_n) _trtc_put(idx, _trtc_val(idx));\n"
+                          "            }\n"
+                          "        }\n"
+                          "    }\n"
+                          "}\n";
+        return get_source_kernel(src, "trtc_map", out);
+    };
+    if (!memo_kernel(memo, build, k)) return false;
     std::vector<ViewBuf> views(names.size());
     std::vector<void*> ptrs(names.size() + 1);
     ptrs[0] = &n;
@@ -145,7 +153,8 @@ bool run_map(const std::vector<const char*>& names, const char* value, const cha
         views[i] = a[i]->view();
         ptrs[i + 1] = views[i].data();
     }
-    return launch_kernel(k, dim3((unsigned)((n + 2047) / 2048)), dim3(256), ptrs.data(), 0, cur_stream());
+    // one 2048-index chunk per workgroup up to 2^31 indices, grid-stride beyond
+    return launch_kernel(k, dim3(capped_grid(n, 2048, (size_t)1 << 20)), dim3(256), ptrs.data(), 0, cur_stream());
 }
 }  // namespace trtc_host
 
@@ -224,9 +233,12 @@ bool TRTC_Replace_Copy(const DVVectorLike& vec_in, DVVectorLike& vec_out, const 
                        const DeviceViewable& new_value)
 {
     if (!need(vec_out, vec_in.size(), "Replace_Copy", "vec_out")) return false;
+    // the element is read once (replace.cpp:27-28), as computed views
+    // (transform, custom, zipped) evaluate their operator on every read
     return run_map({"vin", "vout", "old_value", "new_value"},
-                   "vin[idx] == (decltype(vin)::value_t)old_value ? (decltype(vout)::value_t)new_value\n"
-                   "                                                : (decltype(vout)::value_t)vin[idx]",
+                   "[&] { auto value = vin[idx];\n"
+                   "      return value == (decltype(vin)::value_t)old_value ? (decltype(vout)::value_t)new_value\n"
+                   "                                                        : (decltype(vout)::value_t)value; }()",
                    "vout[idx] = _x;", vec_in.size(), {&vec_in, &vec_out, &old_value, &new_value});
 }
 
@@ -234,8 +246,10 @@ bool TRTC_Replace_Copy_If(const DVVectorLike& vec_in, DVVectorLike& vec_out, con
                           const DeviceViewable& new_value)
 {
     if (!need(vec_out, vec_in.size(), "Replace_Copy_If", "vec_out")) return false;
+    // one read of the element (replace.cpp:39-40)
     return run_map({"vin", "vout", "pred", "new_value"},
-                   "pred(vin[idx]) ? (decltype(vout)::value_t)new_value : (decltype(vout)::value_t)vin[idx]",
+                   "[&] { auto value = vin[idx];\n"
+                   "      return pred(value) ? (decltype(vout)::value_t)new_value : (decltype(vout)::value_t)value; }()",
                    "vout[idx] = _x;", vec_in.size(), {&vec_in, &vec_out, &pred, &new_value});
 }
 

@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdint>
+#include <functional>
 #include <cstdio>
 #include <mutex>
 #include <string>
@@ -56,6 +58,9 @@ struct Context
     size_t lds_per_block = 0;
     bool verbose = false;
     bool kernel_debug = false;
+    // bumped whenever code blocks, headers, include directories or constants
+    // change (they are part of every generated kernel's source)
+    std::atomic<unsigned long long> generation{0};
 
     std::mutex mtx;
     std::vector<std::string> include_dirs;
@@ -121,6 +126,11 @@ bool get_template_kernel(const std::string& name_expr, const std::string& type_t
 bool find_template_kernel(const std::string& name_expr, const std::string& type_text, KernelRef& out);
 // An extern "C" kernel named `fn_name` defined in `kernel_src` (user kernels).
 bool get_source_kernel(const std::string& kernel_src, const std::string& fn_name, KernelRef& out);
+// A kernel resolved once per `key` (plus the context generation) by `build`;
+// later calls with the same key return it without running `build`.
+bool memo_kernel(const std::string& key, const std::function<bool(KernelRef&)>& build, KernelRef& out);
+// Grid of a grid-stride launch over `items` units of `per_block` each, at most `cap` blocks.
+unsigned capped_grid(size_t items, size_t per_block, size_t cap);
 // Compile only (no device needed): for offline cache warm-up and checks.
 bool compile_only(const std::string& arch, const std::string& name_expr, const std::string& type_text,
                   std::string* log);
