@@ -230,3 +230,89 @@ def test_batched_segments_4096_workload(trtc, wl, schedule):
         assert not bad, f"{len(bad)} segments differ from the oracle, first: {bad[:5]}"
     finally:
         trtc.Set_Batch_Schedule(None)
+
+
+# ---------------------------------------------------------------- long-tile pass
+# Sorts of at least 3 * 2^26 4-byte keys with 4-byte or no values take the
+# 1024 x 24-key tile pass.  Its key kinds, descending order, keys-only form,
+# partial last tile and skipped constant digits are checked here by the
+# properties that define a stable sort, in O(n) host work: the output keys are
+# ordered, the payload is a permutation of the indices, every output key is
+# the input key its payload names, and equal keys keep increasing payloads.
+N_LONG = (3 << 26) + 12345
+
+
+def order_image(keys, greater):
+    """Monotone unsigned image of the sort order (-0.0 == +0.0 for floats)."""
+    if keys.dtype == np.float32:
+        u = keys.view(np.uint32).astype(np.int64)
+        neg = u >= (1 << 31)
+        img = np.where(neg, (1 << 31) - (u - (1 << 31)), u + (1 << 31))  # -0 and +0 -> 2^31
+    elif keys.dtype == np.int32:
+        img = keys.astype(np.int64) + (1 << 31)
+    else:
+        img = keys.astype(np.int64)
+    return -img if greater else img
+
+
+def check_stable_sort(keys_in, keys_out, perm, greater):
+    n = len(keys_in)
+    seen = np.zeros(n, dtype=bool)
+    seen[perm] = True
+    assert seen.all(), "payload is not a permutation"
+    assert np.array_equal(keys_out.view(np.uint32), keys_in[perm].view(np.uint32)), "keys do not follow the payload"
+    img = order_image(keys_out, greater)
+    d = np.diff(img)
+    assert (d >= 0).all(), "output not ordered"
+    ties = d == 0
+    assert (np.diff(perm.astype(np.int64))[ties] > 0).all(), "equal keys lost their input order"
+
+
+@pytest.mark.parametrize("case", ["u32_greater", "i32_less", "f32_greater", "u32_below_2p16"])
+def test_long_tile_pass_sort_by_key(trtc, wl, case):
+    say(f"long-tile pass: {case}, n = {N_LONG}")
+    n = N_LONG
+    if case == "u32_greater":
+        keys, greater = wl.u32_uniform(n, 31), True
+    elif case == "i32_less":
+        keys, greater = wl.i32_range(n, 32, -1000000, 1000000), False
+    elif case == "f32_greater":
+        keys, greater = wl.f32_normal_signed_zeros(n, 33, 34), True
+    else:
+        keys, greater = wl.u32_uniform(n, 35) & np.uint32(0xFFFF), False
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv, trtc.Greater() if greater else None)
+    ko, vo = dk.to_host(), dv.to_host()
+    del dk, dv
+    check_stable_sort(keys, ko, vo, greater)
+    if case == "u32_greater":
+        # the keys-only form gives the same keys
+        dk2 = trtc.device_vector_from_numpy(keys)
+        trtc.Sort(dk2, trtc.Greater())
+        assert np.array_equal(dk2.to_host(), ko)
+
+
+def test_long_tile_pass_wide_status(trtc, wl):
+    """2^30 + 7 u32 pairs: the long-tile pass with 64-bit lookback status words
+    (sorts of 2^30 keys or more); the stable-sort properties are checked in
+    chunks of 2^26 to bound host memory."""
+    n = (1 << 30) + 7
+    say(f"long-tile pass, 64-bit status: n = {n}")
+    keys = wl.u32_uniform(n, 36)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv)
+    ko, vo = dk.to_host(), dv.to_host()
+    del dk, dv
+    seen = np.zeros(n, dtype=bool)
+    step = 1 << 26
+    for b in range(0, n, step):
+        e = min(n, b + step + 1)  # one element of overlap joins the chunks
+        k, v = ko[b:e], vo[b:e]
+        seen[v] = True
+        assert np.array_equal(k, keys[v]), f"keys do not follow the payload near {b}"
+        d = np.diff(k.astype(np.int64))
+        assert (d >= 0).all(), f"output not ordered near {b}"
+        assert (np.diff(v.astype(np.int64))[d == 0] > 0).all(), f"equal keys lost their order near {b}"
+    assert seen.all(), "payload is not a permutation"

@@ -572,6 +572,270 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
+// Digit pass for large single sorts of 4-byte keys with 4-byte or no values,
+// on longer tiles (BLOCK x ITEMS = 1024 x 24 keys: 96-key digit runs on
+// average against 64, fewer partially written cache lines at run ends; 2^28
+// u32/u32 pass 1.16 -> 1.10 ms) that fit one workgroup per CU only because
+// keys and values share one LDS staging buffer and the keys wait in LDS, not
+// in registers, while they are ranked:
+//   1. load the keys, compute digits, park the keys in LDS at their input slot,
+//   2. rank inside each wave (8-ballot match, as radix_pass_core),
+//   3. publish the counts, local digit scan, first lookback reads,
+//   4. read the own keys back and write them at their digit-sorted slot,
+//   5. load the values (in flight during the lookback), finish the lookback,
+//   6. write the keys out; stage the values in the same buffer and write them out.
+// Stable: ranks follow tile positions exactly as in radix_pass_core.
+template <int BLOCK, int ITEMS, int VB, class S, int KIND>
+TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
+{
+    typedef unsigned U;
+    typedef unsigned V;
+    typedef S P;
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int TILE = BLOCK * ITEMS;
+    constexpr int LBB = 4;
+    typedef StatusBits<S> SB;
+    static_assert(VB == 0 || VB == 4, "4-byte values or none");
+    static_assert(TILE <= 65536 && ITEMS % 8 == 0, "16-bit local positions, digits packed by four");
+
+    const void* kin_p = a.keys_in;
+    const void* vin_p = a.vals_in;
+    void* kout_p = a.keys_out;
+    void* vout_p = a.vals_out;
+    if (a.plan) {
+        const RadixPlan& PL = *a.plan;
+        if (!PL.kin[a.pass]) return;
+        kin_p = (const void*)PL.kin[a.pass];
+        vin_p = (const void*)PL.vin[a.pass];
+        kout_p = (void*)PL.kout[a.pass];
+        vout_p = (void*)PL.vout[a.pass];
+    }
+
+    __shared__ unsigned s_wcnt[WAVES][256];
+    __shared__ P s_base[256];
+    __shared__ unsigned s_dscan[4];
+    __shared__ unsigned s_tile;
+    __shared__ __attribute__((aligned(16))) unsigned s_stage[TILE];
+
+    for (int i = threadIdx.x; i < WAVES * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    lds_barrier();
+
+    const size_t tile = s_tile;
+    const unsigned wave = threadIdx.x / 64;
+    const unsigned lane = lane_id();
+    const size_t n = a.n;
+    const size_t tile_base = tile * (size_t)TILE;
+    const bool full = tile_base + TILE <= n;
+    const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
+    const U* kin = (const U*)kin_p + tile_base;
+    const V* vin = (const V*)vin_p + tile_base;
+    const int wpos0 = (int)wave * 64 * ITEMS;
+    const int shift = a.shift;
+    const int kind = a.kind;
+    const U flip = a.descending ? ~0u : 0u;
+    S* st = (S*)a.status;
+    const bool head = tile == 0;
+
+    // 1. keys -> digits; keys parked in LDS (each wave in its own region)
+    unsigned dig[ITEMS];
+    {
+        U key[ITEMS];
+        unsigned in_range = ~0u;  // bit k: item k lies inside the input
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (wpos0 + k * 64) + lane);
+        } else {
+            in_range = 0;
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+                key[k] = p < valid ? kin[p] : 0u;
+                in_range |= (p < valid ? 1u : 0u) << k;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            // positions past the end rank as the largest sort key (digit 255) and are never written
+            U sk = sort_key_k<KIND, U>(key[k], kind, flip);
+            if (!full && !((in_range >> k) & 1u)) sk = ~0u;
+            dig[k] = (sk >> shift) & 255u;
+            s_stage[wpos0 + k * 64 + lane] = key[k];
+        }
+        // the parking stores stay here (sunk below the ranking they would keep every key live)
+        __asm__ volatile("" ::: "memory");
+    }
+
+    // 2. in-wave ranking (see radix_pass_core)
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const unsigned d = dig[k];
+        int m[8];
+        unsigned long long bal[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
+            __asm__("" : "+v"(m[b]));
+            bal[b] = ballot(m[b] != 0);
+        }
+        unsigned plo = ~0u, phi = ~0u;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
+            phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
+        }
+        const unsigned below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+        const unsigned cnt = (unsigned)__builtin_popcount(plo) + (unsigned)__builtin_popcount(phi);
+        unsigned* ctr = &s_wcnt[wave][d];
+        if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        dig[k] |= (after - cnt + below) << 8;
+        if (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    lds_barrier();
+
+    // 3. counts, first lookback reads, local digit scan
+    S w[LBB];
+    unsigned dstart = 0, real = 0;
+    if (threadIdx.x < 256) {
+        const unsigned d = threadIdx.x;
+#pragma unroll
+        for (int b = 0; b < LBB; b++) {
+            long long j = (long long)tile - 1 - b;
+            w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + d) : (S)0;
+        }
+        unsigned run = 0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) {
+            unsigned c = s_wcnt[ww][d];
+            s_wcnt[ww][d] = run;
+            run += c;
+        }
+        real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
+        st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+        Plus plus;
+        unsigned inc = wave_inclusive_scan(run, plus);
+        if (lane == 63) s_dscan[wave] = inc;
+        dstart = inc - run;
+    }
+    lds_barrier();
+    if (threadIdx.x < 256) {
+        const unsigned d = threadIdx.x;
+        for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ww++) s_wcnt[ww][d] += dstart;
+    }
+    lds_barrier();
+
+    // 4. local positions (two 16-bit halves per register); own keys read back,
+    //    then written at their digit-sorted slot
+    unsigned lp2[ITEMS / 2];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const unsigned lpos = s_wcnt[wave][dig[k] & 255u] + (dig[k] >> 8);
+        if (k & 1) lp2[k / 2] |= lpos << 16;
+        else lp2[k / 2] = lpos;
+    }
+    __asm__ volatile("" ::: "memory");
+    {
+        U key[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) key[k] = s_stage[wpos0 + k * 64 + lane];
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = key[k];
+    }
+    // the value staging recomputes its addresses from the packed positions
+#pragma unroll
+    for (int i = 0; i < ITEMS / 2; i++) __asm__ volatile("" : "+v"(lp2[i]));
+    __asm__ volatile("" ::: "memory");
+
+    // 5. values in flight during the lookback
+    V val[VB ? ITEMS : 1];
+    if (VB) {
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (wpos0 + k * 64) + lane);
+        } else {
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+                val[k] = p < valid ? vin[p] : 0u;
+            }
+        }
+    }
+    if (threadIdx.x < 256) {
+        const unsigned d = threadIdx.x;
+        unsigned long long excl = 0;
+        if (!head) {
+            long long j = (long long)tile - 1;
+            bool first = true;
+            while (true) {
+                S ws[LBB];
+#pragma unroll
+                for (int b = 0; b < LBB; b++)
+                    ws[b] = first ? w[b] : ((j - b >= 0) ? ld_relaxed(st + (size_t)(j - b) * 256 + d) : (S)0);
+                first = false;
+                bool done = false;
+                int used = 0;
+#pragma unroll
+                for (int b = 0; b < LBB; b++) {
+                    if (done || used != b) break;
+                    S x = ws[b];
+                    if ((x & ~SB::mask) == 0) break;
+                    excl += (unsigned long long)(x & SB::mask);
+                    used = b + 1;
+                    if (x & SB::incl) done = true;
+                }
+                if (done) break;
+                if (used == 0) __builtin_amdgcn_s_sleep(1);
+                j -= used;
+            }
+            st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
+        }
+        s_base[d] = (P)(a.digit_start[d] + excl - dstart);
+    }
+    lds_barrier();
+
+    // 6. keys out (digits remembered, four per register), values staged and out
+    U* kout = (U*)kout_p;
+    V* vout = (V*)vout_p;
+    unsigned dg4[ITEMS / 4];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
+        const U kb = s_stage[j];
+        const unsigned d = (sort_key_k<KIND, U>(kb, kind, flip) >> shift) & 255u;
+        if (r % 4 == 0) dg4[r / 4] = d;
+        else dg4[r / 4] |= d << (8 * (r % 4));
+        const P gpos = s_base[d] + j;
+        // gpos < n by construction; the check keeps a logic error from writing outside the output
+        if ((full || j < valid) && gpos < (P)n) kout[gpos] = kb;
+        if (r % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (!VB) return;
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+        if (full || p < valid) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = val[k];
+    }
+    lds_barrier();
+    // slot indices and digit addresses recomputed here, not kept live from the key write-out
+    unsigned tid2 = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid2));
+#pragma unroll
+    for (int i = 0; i < ITEMS / 4; i++) __asm__ volatile("" : "+v"(dg4[i]));
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const unsigned j = (unsigned)(r * BLOCK) + tid2;
+        const V vb = s_stage[j];
+        const unsigned d = (dg4[r / 4] >> (8 * (r % 4))) & 255u;
+        const P gpos = s_base[d] + j;
+        if ((full || j < valid) && gpos < (P)n) vout[gpos] = vb;
+    }
+}
+
 // The copy a plan may ask for after the passes (the result of an odd number
 // of in-place passes lies in the ping-pong buffers): grid-stride, 16 bytes
 // per step when both buffers and the length allow, else 4 or 1.  Returns at

@@ -152,9 +152,17 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     // 2^22 0.189 -> 0.168 ms, 2^24 0.453 -> 0.414, 2^28 5.96 -> 4.94 against
     // 512 threads; equal at 2^21, 512 faster below (2^20 0.131 vs 0.140 ms).
     const int ITEMS = (KB > 4 || val_bytes > 4) ? 8 : 16;
-    const unsigned kRadixBlock =
-        n / (1024 * (size_t)ITEMS) >= (size_t)std::max(1, context().num_cus) ? 1024u : 512u;
-    const size_t TILE = (size_t)kRadixBlock * ITEMS;
+    const size_t cus = (size_t)std::max(1, context().num_cus);
+    const unsigned kRadixBlock = n / (1024 * (size_t)ITEMS) >= cus ? 1024u : 512u;
+    // 4-byte keys with 4-byte or no values: the long-tile pass
+    // (trtc_radix_pass_park_k, TRTC_PARK_BLOCK x TRTC_PARK_ITEMS = 1024 x 24
+    // keys in the device header) for the largest sorts only.  Whole u32/u32
+    // sorts, 16384-key tiles against 24576-key tiles, one box: 2^24 0.339 vs
+    // 0.357 ms, 2^25 0.629 vs 0.661, 2^26 1.26 vs 1.25, 2^27 2.41 vs 2.38,
+    // 2^28 4.87 vs 4.64
+    constexpr size_t kParkTile = 1024 * 24;
+    const bool park = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n >= (3ull << 26) && n / kParkTile >= cus;
+    const size_t TILE = park ? kParkTile : (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
     const bool wide = n >= (1ull << 30);
     const size_t sw = wide ? 8 : 4;
@@ -180,7 +188,7 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     KernelRef hk;
     std::string hname = "trtc_radix_hist_k<" + std::to_string(KB) + ">";
     if (!get_template_kernel(hname, "", hk)) return false;
-    trtc::RadixHistArgs ha;
+    trtc::RadixHistArgs ha{};
     ha.keys = keys_src;
     ha.n = n;
     ha.kind = kind;
@@ -215,11 +223,15 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     }
 
     KernelRef pk;
-    std::string pname = "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
-                        (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ", " +
-                        std::to_string(kRadixBlock == 1024 ? kind : -1) + ">";
+    std::string pname =
+        park ? "trtc_radix_pass_park_k<" + std::to_string(val_bytes) + ", " +
+                   (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kind) + ">"
+             : "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
+                   (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ", " +
+                   std::to_string(kRadixBlock == 1024 ? kind : -1) + ">";
+    const unsigned pass_block = park ? 1024u : kRadixBlock;
     if (!get_template_kernel(pname, "", pk)) return false;
-    trtc::RadixPassArgs pa;
+    trtc::RadixPassArgs pa{};
     void* kbuf[2] = {keys, ktmp.p};
     void* vbuf[2] = {vals, vtmp.p};
     for (int p = 0; p < KB; p++) {
@@ -237,7 +249,7 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         pa.plan = plan;
         pa.pass = p;
         void* args[] = {&pa};
-        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(kRadixBlock), args, 0, s)) return false;
+        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(pass_block), args, 0, s)) return false;
     }
     // the copy the plan may need (an odd number of in-place passes ran, or
     // none ran out of place); its workgroups return at once otherwise, so the

@@ -253,7 +253,7 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
     for (const auto& C : plan.cls) {
         if (!C.num_tiles) continue;
         if (!fill_u32(scratch + C.off_zero, C.zero_bytes / 4, 0, s)) return false;
-        trtc::SegRadixArgs a;
+        trtc::SegRadixArgs a{};
         a.descs = (const trtc::SegSortDesc*)(scratch + C.off_descs);
         a.tile_seg = (const unsigned*)(scratch + C.off_tile_seg);
         a.num_tiles = C.num_tiles;
