@@ -684,8 +684,10 @@ def main():
     gkeys = N / sec / 1e9
     log(f"Sort_By_Key 2^{a.n} u32 uniform: {ms:.3f} ms/step, {gkeys:.2f} Gkeys/s, kernels: {prof}")
 
-    # dominant kernel: one onesweep digit pass (reads 8 B/pair, writes 8 B/pair)
-    pass_name = [k for k in prof if k.startswith("trtc_radix_pass_k<4, 4,")][0]
+    # dominant kernel: one onesweep digit pass (reads 8 B/pair, writes 8 B/pair);
+    # from 3*2^26 keys the long-tile pass, below it the 1024x16 pass
+    pass_name = max((k for k in prof if k.startswith(("trtc_radix_pass_park_k<4,", "trtc_radix_pass_k<4, 4,"))),
+                    key=lambda k: prof[k]["total_ms"])
     pass_ms = prof[pass_name]["total_ms"] / prof[pass_name]["count"]
     pass_bytes = 16.0 * N
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
