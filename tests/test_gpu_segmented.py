@@ -201,11 +201,12 @@ def test_tuner_candidates_are_all_tested(trtc, wl, vals):
 
 
 # ---------------------------------------------------------------- scan
-@pytest.mark.parametrize("elem", ["int32_t", "uint32_t", "int64_t", "float"])
+@pytest.mark.parametrize("elem", ["int32_t", "uint32_t", "int64_t", "float", "double"])
 @pytest.mark.parametrize("exclusive", [False, True])
 def test_segmented_scan(trtc, orc, wl, elem, exclusive):
     say(f"segmented scan {elem} exclusive={exclusive}")
-    offs = offsets_of(edge_sizes())
+    # edges of the 16384-element tiles (512 x 32 for 4-byte, 1024 x 16 for 8-byte elements)
+    offs = offsets_of(edge_sizes() + [32767, 32768, 32769, 3 * 16384, 3 * 16384 + 1])
     n = int(offs[-1])
     # float values in {-1, 0, 1}: every prefix stays an exact integer
     lo, hi = (-1, 1) if elem == "float" else (-1000, 1000)

@@ -28,14 +28,11 @@ struct RegSeg
         reg_seg_hist<4>();
         reg_seg_hist<8>();
         for (const char* op : {"Plus", "Maximum", "Minimum"}) {
-            reg_seg_scan<int32_t, 16>("int32_t", op);
-            reg_seg_scan<int32_t, 8>("int32_t", op);
-            reg_seg_scan<uint32_t, 16>("uint32_t", op);
-            reg_seg_scan<uint32_t, 8>("uint32_t", op);
-            reg_seg_scan<float, 16>("float", op);
-            reg_seg_scan<float, 8>("float", op);
-            reg_seg_scan<int64_t, 8>("int64_t", op);
-            reg_seg_scan<double, 8>("double", op);
+            reg_seg_scan<int32_t, 32>("int32_t", op);
+            reg_seg_scan<uint32_t, 32>("uint32_t", op);
+            reg_seg_scan<float, 32>("float", op);
+            reg_seg_scan<int64_t, 16>("int64_t", op);
+            reg_seg_scan<double, 16>("double", op);
         }
         reg_seg_reduce<int32_t>("int32_t");
         reg_seg_reduce<uint32_t>("uint32_t");

@@ -42,17 +42,10 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     T* lds = (T*)s_raw[wave];
     ST* ldss = (ST*)s_raw[wave];
 
-    // contiguous aligned views: striped 16-byte vector loads
-    constexpr bool DVI = DirectVec<In, T, ITEMS>::value;
-    constexpr bool DVS = DirectVec<Sten, ST, ITEMS>::value;
-    bool direct_in = false, direct_st = false;
-    if constexpr (DVI) direct_in = full && (((size_t)Contig<In>::ptr(in)) & 15) == 0;
-    if constexpr (DVS) direct_st = full && HAS_ST && (((size_t)Contig<Sten>::ptr(sten)) & 15) == 0;
+    // contiguous views of 4/8-byte elements: 16-byte pieces
     T x[ITEMS];
     int nvalid = ITEMS;
-    if constexpr (DVI) {
-        if (direct_in) load_striped_vec<ITEMS>(Contig<In>::ptr(in) + wave_base, lds, x);
-    }
+    const bool direct_in = load_blocked_vec<ITEMS>(in, wave_base, full, lds, x);
     if (!direct_in) {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
@@ -72,9 +65,7 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
     unsigned cnt = 0;
     if constexpr (HAS_ST) {
         ST sv[ITEMS];
-        if constexpr (DVS) {
-            if (direct_st) load_striped_vec<ITEMS>(Contig<Sten>::ptr(sten) + wave_base, ldss, sv);
-        }
+        const bool direct_st = load_blocked_vec<ITEMS>(sten, wave_base, full, ldss, sv);
         if (!direct_st) {
             wave_sync();
 #pragma unroll

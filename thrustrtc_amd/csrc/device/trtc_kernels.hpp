@@ -245,9 +245,9 @@ __global__ void __launch_bounds__(TRTC_HIST_BLOCK) trtc_seg_radix_hist_k(trtc::S
 }
 
 template <class Op, class T, int ITEMS>
-__global__ void __launch_bounds__(TRTC_BLOCK) trtc_seg_scan_k(Op op, T init, trtc::SegScanArgs a)
+__global__ void __launch_bounds__(ScanBlock<T>::value) trtc_seg_scan_k(Op op, T init, trtc::SegScanArgs a)
 {
-    trtc::seg_scan_tile<TRTC_BLOCK, ITEMS, T>(op, init, a);
+    trtc::seg_scan_tile<ScanBlock<T>::value, ITEMS, T>(op, init, a);
 }
 
 // 16384 elements per reduce tile (4 waves x 4096, chunks of 64*ITEMS)

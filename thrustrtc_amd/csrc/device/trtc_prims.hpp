@@ -307,19 +307,80 @@ template <int ITEMS, class T> TRTC_DEV void load_striped_vec(const T* p, T* lds,
     for (int j = 0; j < ITEMS; j++) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
 }
 
-// Blocked load of a wave's 64*ITEMS items: striped 16-byte vector loads for
-// contiguous views of 4/8-byte elements, else coalesced striped element
-// loads, both transposed through the wave's LDS buffer `lds`.
+// As load_striped_vec for a p aligned to its element size but not to 16
+// bytes: the 16-byte pieces covering [p, p + 64*ITEMS) are loaded (one piece
+// more, read by lane 0) and each element lands in the LDS buffer at its
+// offset from p.  The elements read before p and after the range share a
+// 16-byte piece, hence a page, with an element of the range; they are
+// dropped.  Sequences starting one element past a 16-byte boundary took
+// twice as long with element loads (a 3.9e8-int32 scan 0.74 -> 1.42 ms).
+template <int ITEMS, class T> TRTC_DEV void load_striped_vec_shifted(const T* p, T* lds, T (&x)[ITEMS])
+{
+    constexpr int PER = 16 / (int)sizeof(T);
+    constexpr int NV = ITEMS / PER;
+    constexpr int ELEMS = 64 * ITEMS;
+    static_assert(NV * PER == ITEMS, "whole 16-byte pieces per lane");
+    const unsigned lane = lane_id();
+    const int m = (int)(((size_t)p & 15) / sizeof(T));  // 1 .. PER-1 elements past the boundary
+    const trtc_v4u* q = (const trtc_v4u*)((const char*)p - (size_t)m * sizeof(T));
+    trtc_v4u v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(q + k * 64 + lane);
+    trtc_v4u last = {0u, 0u, 0u, 0u};
+    if (lane == 0) last = __builtin_nontemporal_load(q + NV * 64);
+    wave_sync();  // the buffer's previous contents are consumed
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        T t[PER];
+        __builtin_memcpy(t, &v[k], 16);
+        const int i = PER * (k * 64 + (int)lane) - m;
+#pragma unroll
+        for (int e = 0; e < PER; e++)
+            if (i + e >= 0) lds[WaveTile<ITEMS>::pad(i + e)] = t[e];
+    }
+    if (lane == 0) {
+        T t[PER];
+        __builtin_memcpy(t, &last, 16);
+        const int i = ELEMS - m;
+#pragma unroll
+        for (int e = 0; e < PER; e++)
+            if (e < m) lds[WaveTile<ITEMS>::pad(i + e)] = t[e];
+    }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
+}
+
+// Blocked load of a full wave range through 16-byte pieces when the view
+// holds 4- or 8-byte elements contiguously (aligned or shifted); returns
+// false, having loaded nothing, for any other view, a partial range or a
+// pointer not aligned to its element size.
+template <int ITEMS, class T, class View>
+TRTC_DEV bool load_blocked_vec(const View& v, size_t wave_base, bool full, T* lds, T (&x)[ITEMS])
+{
+    if constexpr (DirectVec<View, T, ITEMS>::value) {
+        if (!full) return false;
+        const T* p = Contig<View>::ptr(v) + wave_base;
+        if ((((size_t)p) & 15) == 0) {
+            load_striped_vec<ITEMS>(p, lds, x);
+            return true;
+        }
+        if ((((size_t)p) % sizeof(T)) == 0) {
+            load_striped_vec_shifted<ITEMS>(p, lds, x);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Blocked load of a wave's 64*ITEMS items: 16-byte pieces for contiguous
+// views of 4/8-byte elements, else coalesced striped element loads, both
+// transposed through the wave's LDS buffer `lds`.
 template <int ITEMS, class T, class View>
 TRTC_DEV void load_blocked(View& v, size_t wave_base, size_t n, bool full, int nvalid, T* lds, T (&x)[ITEMS])
 {
     const unsigned lane = lane_id();
-    if constexpr (DirectVec<View, T, ITEMS>::value) {
-        if (full && (((size_t)Contig<View>::ptr(v)) & 15) == 0) {
-            load_striped_vec<ITEMS>(Contig<View>::ptr(v) + wave_base, lds, x);
-            return;
-        }
-    }
+    if (load_blocked_vec<ITEMS>(v, wave_base, full, lds, x)) return;
     wave_sync();  // the buffer's previous contents are consumed
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {

@@ -137,6 +137,66 @@ template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigne
     P.copy_vdst = need_copy && a.val_bytes ? vd : 0;
 }
 
+// Counts keys[0, n) into bin-major LDS counter columns (hcol = the
+// calling lane's column): thread tid of nthreads takes the 16-byte vectors
+// tid, tid + nthreads, ... of the 16-byte-aligned interior, UNR nontemporal
+// loads per group with the next group's loads issued before the current
+// group is counted (one group at a time leaves the read latency-bound); the
+// keys before the first 16-byte boundary and after the last one are counted
+// one per thread.
+template <int KB, int UNR>
+TRTC_DEV void hist_count_keys(const typename KeyWord<KB>::U* keys, size_t n, size_t tid, size_t nthreads,
+                              unsigned* hcol, int kind, typename KeyWord<KB>::U flip)
+{
+    typedef typename KeyWord<KB>::U U;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    constexpr int PER_VEC = 16 / KB;
+    constexpr int C = KB <= 4 ? 32 : 16;
+    auto count1 = [&](U key) {
+        const U k = sort_key<U>(key, kind, flip);
+#pragma unroll
+        for (int p = 0; p < KB; p++)
+            __hip_atomic_fetch_add(hcol + (p * 256 + ((k >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto countv = [&](const v4u& x) {
+        U ks[PER_VEC];
+        __builtin_memcpy(ks, &x, 16);
+#pragma unroll
+        for (int e = 0; e < PER_VEC; e++) count1(ks[e]);
+    };
+    // keys are aligned to their size, so the head is a whole number of keys
+    const size_t mis = ((size_t)keys & 15) / sizeof(U);
+    size_t head = mis ? (size_t)PER_VEC - mis : 0;
+    if (head > n) head = n;
+    if (tid < head) count1(keys[tid]);
+    const v4u* kv = (const v4u*)(keys + head);
+    const size_t nvec = (n - head) / PER_VEC;
+    const v4u zero = {0u, 0u, 0u, 0u};
+    v4u cur[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+        const size_t i = tid + (size_t)u * nthreads;
+        cur[u] = i < nvec ? __builtin_nontemporal_load(kv + i) : zero;
+    }
+    for (size_t v = tid; v < nvec; v += (size_t)UNR * nthreads) {
+        const size_t vn = v + (size_t)UNR * nthreads;
+        v4u nxt[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const size_t i = vn + (size_t)u * nthreads;
+            nxt[u] = i < nvec ? __builtin_nontemporal_load(kv + i) : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            if (v + (size_t)u * nthreads < nvec) countv(cur[u]);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) cur[u] = nxt[u];
+    }
+    const size_t t0 = head + nvec * PER_VEC;  // fewer than PER_VEC keys follow
+    if (t0 + tid < n) count1(keys[t0 + tid]);
+}
+
 // One read of the keys for all digit histograms (16-byte nontemporal loads).
 // Also clears the passes' lookback state (grid-stride), so no separate
 // clearing launch is needed.  The last workgroup to finish turns the global
@@ -153,7 +213,6 @@ template <int BLOCK, int KB>
 TRTC_DEV void radix_hist(const RadixHistArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
-    constexpr int PER_VEC = 16 / KB;  // keys per 16-byte load
     // Counters bin-major with C columns: lane l counts into column l % C, so
     // the 32 lanes an LDS instruction serves at once hit 32 different banks
     // whatever their digits (C = 16 for 8-byte keys: 128 KB of LDS at most).
@@ -166,47 +225,9 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
 
     unsigned* const hcol = s_h + (lane_id() & (unsigned)(C - 1));
     const U flip = a.descending ? (U)~(U)0 : (U)0;
-    const U* keys = (const U*)a.keys;
     const size_t stride = (size_t)gridDim.x * BLOCK;
     const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    size_t done_keys = 0;
-    if ((((size_t)keys) & 15) == 0) {
-        typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        const v4u* kv = (const v4u*)keys;
-        const size_t nvec = a.n / PER_VEC;
-        auto count = [&](const v4u& x) {
-            U ks[PER_VEC];
-            __builtin_memcpy(ks, &x, 16);
-#pragma unroll
-            for (int e = 0; e < PER_VEC; e++) {
-                U k = sort_key<U>(ks[e], a.kind, flip);
-#pragma unroll
-                for (int p = 0; p < KB; p++)
-                    __hip_atomic_fetch_add(hcol + (p * 256 + ((k >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        };
-        // four independent 16-byte loads in flight per thread before the
-        // counting (one at a time leaves the read latency-bound)
-        constexpr int UNR = 4;
-        size_t v = gid;
-        for (; v + (UNR - 1) * stride < nvec; v += UNR * stride) {
-            v4u x[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; u++) x[u] = __builtin_nontemporal_load(kv + v + u * stride);
-#pragma unroll
-            for (int u = 0; u < UNR; u++) count(x[u]);
-        }
-        for (; v < nvec; v += stride) count(__builtin_nontemporal_load(kv + v));
-        done_keys = nvec * PER_VEC;
-    }
-    for (size_t i = done_keys + gid; i < a.n; i += stride) {
-        U k = sort_key<U>(keys[i], a.kind, flip);
-#pragma unroll
-        for (int p = 0; p < KB; p++)
-            __hip_atomic_fetch_add(hcol + (p * 256 + ((k >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    hist_count_keys<KB, 4>((const U*)a.keys, a.n, gid, stride, hcol, a.kind, flip);
     {
         // clear the passes' lookback state, 16 bytes per store when aligned
         size_t z0 = 0;
@@ -913,7 +934,6 @@ template <int BLOCK, int KB>
 TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
 {
     typedef typename KeyWord<KB>::U U;
-    constexpr int UNROLL = 8;
     // bin-major LDS counters in C columns, lane l counting into column l % C,
     // as in radix_hist
     constexpr int C = KB <= 4 ? 32 : 16;
@@ -930,27 +950,8 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     const size_t ck = a.hist_chunk_keys;
     const size_t base = (size_t)a.hist_chunk[2 * c + 1] * ck;
     const size_t cnt = (d.n - base) < ck ? (size_t)(d.n - base) : ck;
-    const U* keys = (const U*)d.src_keys + base;
     const U flip = a.descending ? (U)~(U)0 : (U)0;
-    for (size_t i0 = 0; i0 < cnt; i0 += (size_t)BLOCK * UNROLL) {
-        U k[UNROLL];
-        bool v[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            size_t i = i0 + (size_t)u * BLOCK + threadIdx.x;
-            v[u] = i < cnt;
-            if (v[u]) k[u] = stream_load(keys + i);
-        }
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            if (!v[u]) continue;
-            U sk = sort_key<U>(k[u], a.kind, flip);
-#pragma unroll
-            for (int p = 0; p < KB; p++)
-                __hip_atomic_fetch_add(hcol + (p * 256 + ((sk >> (8 * p)) & 255)) * C, 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
+    hist_count_keys<KB, 4>((const U*)d.src_keys + base, cnt, threadIdx.x, BLOCK, hcol, a.kind, flip);
     __syncthreads();
     unsigned* hist = a.seg_hist + (size_t)d.multi * KB * 256;
     for (int i = threadIdx.x; i < BINS; i += BLOCK) {

@@ -13,7 +13,7 @@
 // transpose to a blocked layout (lane l holds ITEMS consecutive elements), a
 // sequential thread scan, a wave scan, a workgroup combine, one lookback by
 // wave 0, then the reverse transpose and coalesced striped stores.  A tile
-// reads all of its inputs before it writes any output and touches no other
+// reads all of its inputs before it writes any output and writes no other
 // tile's elements, so in-place scans (in == out) are safe.
 #pragma once
 #include "trtc_prims.hpp"
@@ -47,18 +47,10 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
     const bool full = tile_base + TILE <= n;
     T* lds = s_x[wave];
 
-    // contiguous aligned views: striped 16-byte vector loads
-    constexpr bool DV = DirectVec<Src, T, ITEMS>::value && DirectVec<Out, T, ITEMS>::value;
-    bool direct = false;
-    if constexpr (DV) {
-        const size_t pa = (size_t)Contig<Src>::ptr(src), pb = (size_t)Contig<Out>::ptr(out);
-        direct = full && ((pa | pb) & 15) == 0;
-    }
+    // contiguous views of 4/8-byte elements: 16-byte pieces
     T x[ITEMS];
     int nvalid = ITEMS;
-    if constexpr (DV) {
-        if (direct) load_striped_vec<ITEMS>(Contig<Src>::ptr(src) + wave_base, lds, x);
-    }
+    const bool direct = load_blocked_vec<ITEMS>(src, wave_base, full, lds, x);
     if (!direct) {
     // striped load -> LDS
     if (full) {
