@@ -135,6 +135,25 @@ def test_segmented_sort_keys_only(trtc, orc, wl, schedule):
     check_segments_sorted(orc, keys, dk.to_host(), None, offs, elem, False)
 
 
+@pytest.mark.parametrize("elem", ["uint8_t", "int16_t"])
+@pytest.mark.parametrize("schedule", ["rest:512x16", "rest:256x8"])
+def test_segmented_sort_narrow_keys(trtc, orc, wl, elem, schedule):
+    """1- and 2-byte keys: multi-tile segments start up to 15 / 7 keys past a
+    16-byte boundary, and tile 0 of each gives up that many keys so that the
+    later tiles start aligned."""
+    say(f"segmented sort_by_key {elem} keys schedule={schedule}")
+    trtc.Set_Batch_Schedule(schedule)
+    sizes = [1, 9000, 3, 20000, 7, 8193, 13, 16385, 5, 40000, 2, 2049, 11, 4097]
+    offs = offsets_of(sizes)
+    n = int(offs[-1])
+    raw = wl.u32_uniform(n, 47)
+    keys = raw.astype(np.uint8) if elem == "uint8_t" else (raw >> np.uint32(16)).astype(np.uint16).view(np.int16)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Segmented_Sort_By_Key(dk, dv, offs)
+    check_segments_sorted(orc, keys, dk.to_host(), dv.to_host(), offs, elem, False)
+
+
 def test_segmented_sort_narrow_values(trtc, orc, wl):
     say("segmented sort_by_key u32 keys, int16 values, odd widths")
     trtc.Set_Batch_Schedule(None)

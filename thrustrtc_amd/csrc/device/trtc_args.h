@@ -136,7 +136,8 @@ struct SegSortDesc
     unsigned multi;                         // slot among multi-tile segments, 0xFFFFFFFF if single-tile
     unsigned long long status_row;          // multi-tile: lookback row of tile 0 (rows of 256 words)
     unsigned hist_chunks;                   // multi-tile: histogram workgroups of the segment
-    unsigned pad;
+    unsigned lead;                          // multi-tile: (keys % 128) / key size; tile 0 holds tile - lead
+                                            // keys so that later tiles start on 128-byte lines
 };
 
 struct SegRadixArgs
@@ -164,7 +165,8 @@ struct SegScanDesc
     unsigned long long in, out;  // device addresses
     unsigned long long n;
     unsigned first_tile;
-    unsigned pad;
+    unsigned lead;  // multi-tile: (in % 128) / element size; tile 0 holds tile - lead elements
+                    // so that later tiles start on 128-byte lines
 };
 
 struct SegScanArgs

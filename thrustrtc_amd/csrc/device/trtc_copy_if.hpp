@@ -50,7 +50,9 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             int i = k * 64 + (int)lane;
-            if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)in[wave_base + i];
+            const size_t g = wave_base + i;  // past the end: a clamped index, so the loads issue together
+            const T v = (T)in[full || g < n ? g : n - 1];
+            if (full || g < n) lds[WaveTile<ITEMS>::pad(i)] = v;
         }
         wave_sync();
         if (!full) {
@@ -71,7 +73,9 @@ TRTC_DEV void copy_if_tile(In& in, Sten& sten, Out& out, Pred& pred, const CopyI
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 int i = k * 64 + (int)lane;
-                if (full || wave_base + i < n) ldss[WaveTile<ITEMS>::pad(i)] = sten[wave_base + i];
+                const size_t g = wave_base + i;
+                const ST v = sten[full || g < n ? g : n - 1];
+                if (full || g < n) ldss[WaveTile<ITEMS>::pad(i)] = v;
             }
             wave_sync();
 #pragma unroll

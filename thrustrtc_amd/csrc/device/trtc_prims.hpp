@@ -385,7 +385,11 @@ TRTC_DEV void load_blocked(View& v, size_t wave_base, size_t n, bool full, int n
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         int i = k * 64 + (int)lane;
-        if (full || wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)v[wave_base + i];
+        // past the end a clamped index: a load inside the bounds test would
+        // wait before the next one could issue
+        const size_t g = wave_base + i;
+        const T e = (T)v[full || g < n ? g : n - 1];
+        if (full || g < n) lds[WaveTile<ITEMS>::pad(i)] = e;
     }
     wave_sync();
 #pragma unroll

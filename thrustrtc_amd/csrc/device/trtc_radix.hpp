@@ -139,11 +139,12 @@ template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigne
 
 // Counts keys[0, n) into bin-major LDS counter columns (hcol = the
 // calling lane's column): thread tid of nthreads takes the 16-byte vectors
-// tid, tid + nthreads, ... of the 16-byte-aligned interior, UNR nontemporal
+// tid, tid + nthreads, ... of the interior that starts on a 128-byte line
+// (a wave's load instruction then covers 8 lines, not 9), UNR nontemporal
 // loads per group with the next group's loads issued before the current
 // group is counted (one group at a time leaves the read latency-bound); the
-// keys before the first 16-byte boundary and after the last one are counted
-// one per thread.
+// keys before the first line boundary and after the last 16-byte piece are
+// counted one per thread (nthreads >= 128 / sizeof(key)).
 template <int KB, int UNR>
 TRTC_DEV void hist_count_keys(const typename KeyWord<KB>::U* keys, size_t n, size_t tid, size_t nthreads,
                               unsigned* hcol, int kind, typename KeyWord<KB>::U flip)
@@ -166,8 +167,9 @@ TRTC_DEV void hist_count_keys(const typename KeyWord<KB>::U* keys, size_t n, siz
         for (int e = 0; e < PER_VEC; e++) count1(ks[e]);
     };
     // keys are aligned to their size, so the head is a whole number of keys
-    const size_t mis = ((size_t)keys & 15) / sizeof(U);
-    size_t head = mis ? (size_t)PER_VEC - mis : 0;
+    constexpr size_t LINE_KEYS = 128 / sizeof(U);
+    const size_t mis = ((size_t)keys & 127) / sizeof(U);
+    size_t head = mis ? LINE_KEYS - mis : 0;
     if (head > n) head = n;
     if (tid < head) count1(keys[tid]);
     const v4u* kv = (const v4u*)(keys + head);
@@ -328,6 +330,7 @@ template <class U, class V, class S> struct RadixJob
     S* status;                            // lookback rows of the sequence; null for a one-tile sequence
     const unsigned long long* dstart64;   // [256] digit offsets of the pass, or
     const unsigned* dstart32;             // [256] (null both: one-tile sequence)
+    unsigned lead = 0;                    // keys tile 0 gives up: tile t >= 1 starts at t*TILE - lead
 };
 
 template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND, class MakeJob>
@@ -363,9 +366,13 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     const unsigned wave = threadIdx.x / 64;
     const unsigned lane = lane_id();
     const size_t n = job.n;
-    const size_t tile_base = tile * (size_t)TILE;
-    const bool full = tile_base + TILE <= n;
-    const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
+    // tile 0 holds TILE - lead keys, so that with lead = the sequence's
+    // offset from a 128-byte line in keys every later tile starts on a line
+    // (a 256-byte load instruction then covers two lines, not three)
+    const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
+    const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
+    const unsigned valid = (unsigned)(tile_end - tile_base);
+    const bool full = valid == (unsigned)TILE;
     const U flip = descending ? (U)~(U)0 : (U)0;
     const U* kin = job.kin + tile_base;
     const V* vin = job.vin + tile_base;
@@ -385,11 +392,11 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     } else {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
+            // past the end a clamped index, so that the loads issue together
             unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-            if (p < valid) {
-                key[k] = kin[p];
-                if (VB) val[k] = vin[p];
-            }
+            const unsigned q = p < valid ? p : valid - 1;
+            key[k] = kin[q];
+            if (VB) val[k] = vin[q];
         }
     }
     // 2. digits (positions past the end rank as the largest sort key, digit
@@ -671,7 +678,7 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-                key[k] = p < valid ? kin[p] : 0u;
+                key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
                 in_range |= (p < valid ? 1u : 0u) << k;
             }
         }
@@ -781,7 +788,7 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-                val[k] = p < valid ? vin[p] : 0u;
+                val[k] = vin[p < valid ? p : valid - 1];
             }
         }
     }
@@ -912,6 +919,7 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         j.vout = (V*)(odd ? d.vals : d.tmp_vals);
         j.n = d.n;
         j.tile = t - d.first_tile;
+        j.lead = d.lead;
         j.dstart64 = nullptr;
         if (d.n > (unsigned long long)TILE) {
             j.status = a.status + d.status_row * 256;

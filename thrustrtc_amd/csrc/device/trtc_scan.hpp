@@ -28,10 +28,12 @@ template <class T> struct MaybeT
 
 // One tile of a sequence: lookback index `tile` (status slot), `first` = the
 // lookback index of the sequence's first tile, elements [base, min(base+TILE,
-// n)) of src/out.
+// n)) of src/out.  The loads may read up to `readable` (>= n): a tile that
+// ends early inside its sequence still loads its whole extent through
+// 16-byte pieces and drops the elements past n.
 template <int BLOCK, int ITEMS, class T, class Src, class Out, class Op>
 TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclusive, const LookbackArgs& lb,
-                           size_t tile, size_t first, size_t tile_base, size_t n)
+                           size_t tile, size_t first, size_t tile_base, size_t n, size_t readable)
 {
     constexpr int WAVES = BLOCK / 64;
     constexpr int WELEMS = 64 * ITEMS;
@@ -50,7 +52,11 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
     // contiguous views of 4/8-byte elements: 16-byte pieces
     T x[ITEMS];
     int nvalid = ITEMS;
-    const bool direct = load_blocked_vec<ITEMS>(src, wave_base, full, lds, x);
+    if (!full) {
+        size_t first = wave_base + (size_t)lane * ITEMS;
+        nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
+    }
+    const bool direct = load_blocked_vec<ITEMS>(src, wave_base, tile_base + TILE <= readable, lds, x);
     if (!direct) {
     // striped load -> LDS
     if (full) {
@@ -63,16 +69,16 @@ TRTC_DEV void scan_tile_at(Src& src, Out& out, Op& op, const T& init, int exclus
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             int i = k * 64 + (int)lane;
-            if (wave_base + i < n) lds[WaveTile<ITEMS>::pad(i)] = (T)src[wave_base + i];
+            // past the end a clamped index, so that the loads issue together
+            // (a load inside the bounds test waits before the next one)
+            const size_t g = wave_base + i;
+            const T v = (T)src[g < n ? g : n - 1];
+            if (g < n) lds[WaveTile<ITEMS>::pad(i)] = v;
         }
     }
     wave_sync();
 
     // blocked read: lane holds elements lane*ITEMS .. lane*ITEMS+ITEMS-1
-    if (!full) {
-        size_t first = wave_base + (size_t)lane * ITEMS;
-        nvalid = first >= n ? 0 : (int)((n - first) < (size_t)ITEMS ? (n - first) : ITEMS);
-    }
 #pragma unroll
     for (int j = 0; j < ITEMS; j++)
         if (j < nvalid) x[j] = lds[WaveTile<ITEMS>::pad((int)lane * ITEMS + j)];
@@ -203,7 +209,8 @@ TRTC_DEV void scan_tile(Src& src, Out& out, Op& op, const T& init, const ScanArg
     if (threadIdx.x == 0) s_tile = atomicAdd(a.lb.tile_counter, 1u);
     __syncthreads();
     const size_t tile = s_tile;
-    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, a.lb, tile, 0, tile * (size_t)BLOCK * ITEMS, a.n);
+    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, a.lb, tile, 0, tile * (size_t)BLOCK * ITEMS, a.n,
+                                  a.n);
 }
 
 // One tile of a batch of independent scans: segments of more than one tile
@@ -222,8 +229,11 @@ TRTC_DEV void seg_scan_tile(Op& op, const T& init, const SegScanArgs& a)
     PtrView<T> out{(T*)d.out};
     LookbackArgs lb = a.lb;
     if (d.n <= TILE) lb.incl = nullptr;
+    // tile 0 holds TILE - lead elements, so later tiles start on 128-byte lines
     const size_t lt = tile - d.first_tile;
-    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, lb, tile, (size_t)d.first_tile, lt * TILE,
+    const size_t base = lt ? lt * TILE - d.lead : 0;
+    const size_t end = (lt + 1) * TILE - d.lead < (size_t)d.n ? (lt + 1) * TILE - d.lead : (size_t)d.n;
+    scan_tile_at<BLOCK, ITEMS, T>(src, out, op, init, a.exclusive, lb, tile, (size_t)d.first_tile, base, end,
                                   (size_t)d.n);
 }
 
