@@ -333,7 +333,13 @@ template <class U, class V, class S> struct RadixJob
     unsigned lead = 0;                    // keys tile 0 gives up: tile t >= 1 starts at t*TILE - lead
 };
 
-template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND, class MakeJob>
+// CLAMP_TAIL: the loads of a partial tile read a clamped index (all issued
+// together) instead of testing each item.  Both give the same results; which
+// one the compiler turns into the faster full-tile code differs by caller:
+// clamped, the batched pass over 128 segments of 2^20 u32/u32 pairs runs
+// 0.588 -> 0.550 ms and a single 2^27-pair pass 0.577 -> 0.563 ms, but a
+// single 2^25-pair pass 0.143 -> 0.153 ms.
+template <int BLOCK, int ITEMS, int KB, int VB, class S, int KIND, bool CLAMP_TAIL, class MakeJob>
 TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, int shift, int kind, int descending)
 {
     typedef typename KeyWord<KB>::U U;
@@ -392,11 +398,15 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     } else {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            // past the end a clamped index, so that the loads issue together
             unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-            const unsigned q = p < valid ? p : valid - 1;
-            key[k] = kin[q];
-            if (VB) val[k] = vin[q];
+            if (CLAMP_TAIL) {
+                const unsigned q = p < valid ? p : valid - 1;
+                key[k] = kin[q];
+                if (VB) val[k] = vin[q];
+            } else if (p < valid) {
+                key[k] = kin[p];
+                if (VB) val[k] = vin[p];
+            }
         }
     }
     // 2. digits (positions past the end rank as the largest sort key, digit
@@ -597,7 +607,7 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
         j.dstart32 = nullptr;
         return j;
     };
-    radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
+    radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND, false>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
 // Digit pass for large single sorts of 4-byte keys with 4-byte or no values,
@@ -930,7 +940,8 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         }
         return j;
     };
-    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned, KIND>(a.tile_counter, make, a.shift, a.kind, a.descending);
+    radix_pass_core<BLOCK, ITEMS, KB, VB, unsigned, KIND, true>(a.tile_counter, make, a.shift, a.kind,
+                                                                a.descending);
 }
 
 // Digit histograms of the multi-tile segments of a batch: one workgroup per
