@@ -123,9 +123,11 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         plan.cls[c].segs.push_back(g);
         plan.keys_total += g.n;
     }
-    // histogram chunk: 16K..128K keys (16 to 128 per thread of the 1024-thread
+    // histogram chunk: 16K..1M keys (16 to 1024 per thread of the 1024-thread
     // workgroups), small enough that the batch's multi-tile keys give every
-    // CU a few workgroups
+    // CU about four workgroups; each chunk costs ~12 us of its own (counter
+    // clearing, column sums, global adds), so large batches take large chunks
+    // (a served batch of 1e9 keys: 8227 chunks of 128K keys took 1.2 ms)
     {
         size_t multi_keys = 0;
         for (auto& C : plan.cls)
@@ -133,7 +135,7 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
                 if (g.n > C.tile) multi_keys += g.n;
         size_t want = multi_keys / ((size_t)std::max(1, context().num_cus) * 4);
         size_t ck = 16384;
-        while (ck < want && ck < 131072) ck <<= 1;
+        while (ck < want && ck < ((size_t)1 << 20)) ck <<= 1;
         plan.hist_chunk = ck;
     }
     size_t off = 0;
