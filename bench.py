@@ -686,7 +686,7 @@ def main():
 
     # dominant kernel: one onesweep digit pass (reads 8 B/pair, writes 8 B/pair);
     # from 3*2^26 keys the long-tile pass, below it the 1024x16 pass
-    pass_name = max((k for k in prof if k.startswith(("trtc_radix_pass_park_k<4,", "trtc_radix_pass_k<4, 4,"))),
+    pass_name = max((k for k in prof if k.startswith(("trtc_radix_pass_long_k<4,", "trtc_radix_pass_k<4, 4,"))),
                     key=lambda k: prof[k]["total_ms"])
     pass_ms = prof[pass_name]["total_ms"] / prof[pass_name]["count"]
     pass_bytes = 16.0 * N

@@ -12,12 +12,12 @@ template <int VB, class S> void reg_pass(const char* sn)
     TRTC_REG((name + "1024, 1>").c_str(), (&trtc_radix_pass_k<4, VB, S, 1024, 1>));
     TRTC_REG((name + "1024, 2>").c_str(), (&trtc_radix_pass_k<4, VB, S, 1024, 2>));
 }
-template <int VB, class S> void reg_park(const char* sn)
+template <int VB, class S> void reg_long(const char* sn)
 {
-    std::string name = std::string("trtc_radix_pass_park_k<") + std::to_string(VB) + ", " + sn + ", ";
-    TRTC_REG((name + "0>").c_str(), (&trtc_radix_pass_park_k<VB, S, 0>));
-    TRTC_REG((name + "1>").c_str(), (&trtc_radix_pass_park_k<VB, S, 1>));
-    TRTC_REG((name + "2>").c_str(), (&trtc_radix_pass_park_k<VB, S, 2>));
+    std::string name = std::string("trtc_radix_pass_long_k<") + std::to_string(VB) + ", " + sn + ", ";
+    TRTC_REG((name + "0>").c_str(), (&trtc_radix_pass_long_k<VB, S, 0>));
+    TRTC_REG((name + "1>").c_str(), (&trtc_radix_pass_long_k<VB, S, 1>));
+    TRTC_REG((name + "2>").c_str(), (&trtc_radix_pass_long_k<VB, S, 2>));
 }
 struct RegRadix
 {
@@ -34,10 +34,10 @@ struct RegRadix
         reg_pass<2, unsigned long long>("unsigned long long");
         reg_pass<4, unsigned long long>("unsigned long long");
         reg_pass<8, unsigned long long>("unsigned long long");
-        reg_park<0, unsigned int>("unsigned int");
-        reg_park<4, unsigned int>("unsigned int");
-        reg_park<0, unsigned long long>("unsigned long long");
-        reg_park<4, unsigned long long>("unsigned long long");
+        reg_long<0, unsigned int>("unsigned int");
+        reg_long<4, unsigned int>("unsigned int");
+        reg_long<0, unsigned long long>("unsigned long long");
+        reg_long<4, unsigned long long>("unsigned long long");
     }
 } reg_radix;
 }  // namespace

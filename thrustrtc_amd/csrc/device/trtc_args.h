@@ -53,10 +53,14 @@ struct ScanByKeyArgs
 // pass is skipped), and the copy that brings the result into the caller's
 // buffers when an odd number of passes ran in place (ksrc 0: none).  The
 // passes and the copy read it, so skipping needs no host synchronisation.
+// rank_lds: bit p set when pass p's digits are spread thinly enough over the
+// 256 bins for the long-tile pass to rank them with LDS count|mask words
+// rather than ballots (few lanes of a slot share a digit).
 struct RadixPlan
 {
     unsigned long long kin[8], kout[8], vin[8], vout[8];
     unsigned long long copy_ksrc, copy_kdst, copy_vsrc, copy_vdst;
+    unsigned rank_lds;
 };
 
 struct RadixHistArgs

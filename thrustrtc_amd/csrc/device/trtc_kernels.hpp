@@ -218,14 +218,16 @@ __global__ void __launch_bounds__(BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a
     trtc::radix_pass<BLOCK, RadixItems<KB, VB>::value, KB, VB, S, KIND>(a);
 }
 
-// Long-tile digit pass (1024 x 24 keys) for large sorts of 4-byte keys with
-// 4-byte or no values (trtc::radix_pass_park).
-#define TRTC_PARK_BLOCK 1024
-#define TRTC_PARK_ITEMS 24
+// Long-tile digit pass (1024 x 32 keys, ranked in 32-lane units) for large
+// sorts of 4-byte keys with 4-byte or no values (trtc::radix_pass_long).  The
+// grid is padded to whole runs of 8 tiles per XCD (64 tiles); the padding
+// workgroups return at once.
+#define TRTC_LONG_BLOCK 1024
+#define TRTC_LONG_ITEMS 32
 template <int VB, class S, int KIND>
-__global__ void __launch_bounds__(TRTC_PARK_BLOCK) trtc_radix_pass_park_k(trtc::RadixPassArgs a)
+__global__ void __launch_bounds__(TRTC_LONG_BLOCK) trtc_radix_pass_long_k(trtc::RadixPassArgs a)
 {
-    trtc::radix_pass_park<TRTC_PARK_BLOCK, TRTC_PARK_ITEMS, VB, S, KIND>(a);
+    trtc::radix_pass_long<TRTC_LONG_BLOCK, TRTC_LONG_ITEMS, VB, S, KIND>(a);
 }
 
 // ---- batched segments (one launch over many independent sequences) ----

@@ -104,9 +104,10 @@ TRTC_DEV void st_relaxed(unsigned long long* p, unsigned long long v)
 // pass must leave the source, so an odd number of passes ends in the
 // ping-pong buffers and a copy brings the result back.  No pass runs when
 // every digit is constant (then only an out-of-place sort copies).
-template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigned constant_mask)
+template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigned constant_mask, unsigned rank_lds)
 {
     RadixPlan& P = *a.plan;
+    P.rank_lds = rank_lds;
     const unsigned long long ks = (unsigned long long)a.keys, vs = (unsigned long long)a.vals_src;
     const unsigned long long kd = (unsigned long long)a.keys_dst, vd = (unsigned long long)a.vals_dst;
     const unsigned long long kt = (unsigned long long)a.keys_tmp, vt = (unsigned long long)a.vals_tmp;
@@ -266,23 +267,34 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     // Last workgroup: exclusive scan of each 256-bin histogram; a pass whose
     // digit has one bin holding all n keys is constant (reference
     // radix_sort.cpp:323-325, 351-353: all-equal keys and constant bits).
+    // Also the chance that two keys share the pass's digit, sum (h_d / n)^2:
+    // below 1/32 the long-tile pass ranks with LDS words (a uniform digit:
+    // 1/256), above it with ballots (the LDS words serialise the lanes of a
+    // slot that share a digit: keys that AND three random words, ~0.14).
     if (threadIdx.x < 64) {
-        unsigned constant_mask = 0;
+        unsigned constant_mask = 0, rank_lds = 0;
+        const double inv_n = 1.0 / (double)a.n;
         for (int p = 0; p < KB; p++) {
             unsigned long long run = 0;
             bool whole = false;
+            double sq = 0.0;
             for (int c = 0; c < 4; c++) {
                 int d = c * 64 + (int)threadIdx.x;
                 unsigned long long v = ld_relaxed(a.hist + p * 256 + d);
                 whole = whole || v == (unsigned long long)a.n;
+                const double f = (double)v * inv_n;
+                sq += f * f;
                 Plus plus;
                 unsigned long long inc = wave_inclusive_scan(v, plus);
                 a.digit_start[p * 256 + d] = run + inc - v;
                 run += shfl(inc, 63);
             }
             if (ballot(whole)) constant_mask |= 1u << p;
+            Plus plus;
+            sq = shfl(wave_inclusive_scan(sq, plus), 63);
+            if (sq < 1.0 / 32) rank_lds |= 1u << p;
         }
-        if (a.plan && threadIdx.x == 0) write_radix_plan<KB>(a, constant_mask);
+        if (a.plan && threadIdx.x == 0) write_radix_plan<KB>(a, constant_mask, rank_lds);
     }
 }
 
@@ -610,36 +622,62 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND, false>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
-// Digit pass for large single sorts of 4-byte keys with 4-byte or no values,
-// on longer tiles (BLOCK x ITEMS = 1024 x 24 keys: 96-key digit runs on
-// average against 64, fewer partially written cache lines at run ends; 2^28
-// u32/u32 pass 1.16 -> 1.10 ms) that fit one workgroup per CU only because
-// keys and values share one LDS staging buffer and the keys wait in LDS, not
-// in registers, while they are ranked:
-//   1. load the keys, compute digits, park the keys in LDS at their input slot,
-//   2. rank inside each wave (8-ballot match, as radix_pass_core),
-//   3. publish the counts, local digit scan, first lookback reads,
-//   4. read the own keys back and write them at their digit-sorted slot,
-//   5. load the values (in flight during the lookback), finish the lookback,
-//   6. write the keys out; stage the values in the same buffer and write them out.
-// Stable: ranks follow tile positions exactly as in radix_pass_core.
+// Digit pass for large single sorts of 4-byte keys with 4-byte or no values:
+// BLOCK x ITEMS = 1024 x 32 keys per tile (128-key digit runs on average, so
+// fewer partially written 128-byte lines at run ends than the 64-key runs of
+// 16384-key tiles), ranked in 32-lane units:
+//   * unit u = threadIdx.x / 32 owns the 32*ITEMS consecutive tile positions
+//     from u*32*ITEMS; item k of lane l (l = lane % 32) sits at u*32*ITEMS +
+//     k*32 + l, so a load instruction covers two whole lines, one per unit;
+//   * each unit has 256 LDS words {count:32 | peer mask:32}.  Flat digits
+//     (the plan's rank_lds bit): every lane adds (1 << 32) | (1 << l) to its
+//     digit's word and reads it back; the mask gives the lanes of this slot
+//     with the same digit, the count all such keys so far: rank = count -
+//     popcount(mask) + popcount(mask below l); the lowest such lane clears the
+//     mask.  A wave's LDS operations execute in issue order, so the read sees
+//     this slot's adds and no later one.  Skewed digits: the 8-ballot match of
+//     radix_pass_core on the unit's half of the wave, the lowest peer adding
+//     the peer count to the word's count (LDS adds to one address serialise,
+//     and with ~a third of the keys in one bin the ballots are cheaper).
+//   * ranks are 16-bit, two per register, and digits are recomputed from the
+//     keys where needed, so 32 keys fit 128 VGPRs with the keys in registers;
+//     the rank words live in the staging buffer until the local positions
+//     are known.
+// Tiles go to XCDs in runs of 8 consecutive tiles (per-XCD counters, XCD =
+// blockIdx.x mod 8, the order in which the hardware spreads workgroups): the
+// partial lines that neighbouring tiles write at the two ends of a digit run
+// then meet in one L2 instead of reaching HBM separately.  2^28 u32/u32, one
+// pass, uniform keys: 1.105 ms (1024 x 24, wave ranking, arrival order) ->
+// 0.928 ms; low-entropy keys 1.080 -> 1.009 ms.
+//   1. take a tile (grid padded to whole runs; tiles past the end return),
+//   2. load keys, rank in units,
+//   3. publish the tile's digit counts, local digit scan, first lookback reads,
+//   4. write keys to LDS at their digit-sorted slot; load the values,
+//   5. finish the lookback; write the keys out,
+//   6. stage the values in the same buffer and write them out.
+// Stable: a unit's ranks follow its positions (slot, then lane) and units are
+// scanned in position order.
 template <int BLOCK, int ITEMS, int VB, class S, int KIND>
-TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
+TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
 {
     typedef unsigned U;
     typedef unsigned V;
     typedef S P;
-    constexpr int WAVES = BLOCK / 64;
+    constexpr int UNITS = BLOCK / 32;
     constexpr int TILE = BLOCK * ITEMS;
+    constexpr int UITEMS = 32 * ITEMS;
     constexpr int LBB = 4;
+    constexpr unsigned XRUN = 8;  // consecutive tiles per XCD
     typedef StatusBits<S> SB;
     static_assert(VB == 0 || VB == 4, "4-byte values or none");
-    static_assert(TILE <= 65536 && ITEMS % 8 == 0, "16-bit local positions, digits packed by four");
+    static_assert(TILE * 4 >= UNITS * 256 * 8, "rank words fit the staging buffer");
+    static_assert(TILE <= 65536 && ITEMS % 4 == 0, "16-bit local positions, digits packed by four");
 
     const void* kin_p = a.keys_in;
     const void* vin_p = a.vals_in;
     void* kout_p = a.keys_out;
     void* vout_p = a.vals_out;
+    bool lds_rank = false;
     if (a.plan) {
         const RadixPlan& PL = *a.plan;
         if (!PL.kin[a.pass]) return;
@@ -647,89 +685,128 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
         vin_p = (const void*)PL.vin[a.pass];
         kout_p = (void*)PL.kout[a.pass];
         vout_p = (void*)PL.vout[a.pass];
+        lds_rank = (PL.rank_lds >> a.pass) & 1u;
     }
 
-    __shared__ unsigned s_wcnt[WAVES][256];
+    __shared__ __attribute__((aligned(16))) unsigned s_stage[TILE];
+    __shared__ unsigned short s_off[UNITS][256];  // unit's first local slot per digit
     __shared__ P s_base[256];
     __shared__ unsigned s_dscan[4];
     __shared__ unsigned s_tile;
-    __shared__ __attribute__((aligned(16))) unsigned s_stage[TILE];
+    unsigned long long* const s_rank = (unsigned long long*)s_stage;
 
-    for (int i = threadIdx.x; i < WAVES * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        v4u* z = (v4u*)s_rank;
+        const v4u zero = {0u, 0u, 0u, 0u};
+        for (int i = threadIdx.x; i < UNITS * 256 / 2; i += BLOCK) z[i] = zero;
+    }
+    if (threadIdx.x == 0) {
+        const unsigned x = blockIdx.x & 7u;
+        const unsigned k = atomicAdd(a.tile_counter + x, 1u);
+        s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
+    }
     lds_barrier();
 
     const size_t tile = s_tile;
-    const unsigned wave = threadIdx.x / 64;
-    const unsigned lane = lane_id();
     const size_t n = a.n;
     const size_t tile_base = tile * (size_t)TILE;
+    if (tile_base >= n) return;  // padding of the grid
+    const unsigned wave = threadIdx.x / 64;
+    const unsigned lane = lane_id();
+    const unsigned l32 = lane & 31u;
+    const unsigned unit = threadIdx.x >> 5;
     const bool full = tile_base + TILE <= n;
     const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
     const U* kin = (const U*)kin_p + tile_base;
     const V* vin = (const V*)vin_p + tile_base;
-    const int wpos0 = (int)wave * 64 * ITEMS;
+    const int upos0 = (int)unit * UITEMS;
     const int shift = a.shift;
     const int kind = a.kind;
     const U flip = a.descending ? ~0u : 0u;
     S* st = (S*)a.status;
     const bool head = tile == 0;
 
-    // 1. keys -> digits; keys parked in LDS (each wave in its own region)
-    unsigned dig[ITEMS];
-    {
-        U key[ITEMS];
-        unsigned in_range = ~0u;  // bit k: item k lies inside the input
-        if (full) {
+    // 1. keys (registers)
+    U key[ITEMS];
+    unsigned in_range = ~0u;  // bit k: item k lies inside the input
+    if (full) {
 #pragma unroll
-            for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (wpos0 + k * 64) + lane);
-        } else {
-            in_range = 0;
-#pragma unroll
-            for (int k = 0; k < ITEMS; k++) {
-                unsigned p = (unsigned)(wpos0 + k * 64) + lane;
-                key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
-                in_range |= (p < valid ? 1u : 0u) << k;
-            }
-        }
+        for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (upos0 + k * 32) + l32);
+    } else {
+        in_range = 0;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            // positions past the end rank as the largest sort key (digit 255) and are never written
-            U sk = sort_key_k<KIND, U>(key[k], kind, flip);
-            if (!full && !((in_range >> k) & 1u)) sk = ~0u;
-            dig[k] = (sk >> shift) & 255u;
-            s_stage[wpos0 + k * 64 + lane] = key[k];
+            unsigned p = (unsigned)(upos0 + k * 32) + l32;
+            key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
+            in_range |= (p < valid ? 1u : 0u) << k;
         }
-        // the parking stores stay here (sunk below the ranking they would keep every key live)
-        __asm__ volatile("" ::: "memory");
     }
+    // digit of item k, recomputed from the key wherever it is needed
+    // (positions past the end rank as digit 255 and are never written)
+    auto digit_of = [&](int k) -> unsigned {
+        U sk = sort_key_k<KIND, U>(key[k], kind, flip);
+        if (!full && !((in_range >> k) & 1u)) sk = ~0u;
+        return (sk >> shift) & 255u;
+    };
+    unsigned rk2[ITEMS / 2];  // ranks inside the unit, two 16-bit halves per register
+    // the asm keeps each rank's arithmetic inside its slot rather than holding
+    // every slot's LDS words until the end of the loop
+    auto put_rank = [&](int k, unsigned r) {
+        if (k & 1) rk2[k / 2] |= r << 16;
+        else rk2[k / 2] = r;
+        __asm__ volatile("" : "+v"(rk2[k / 2]));
+    };
 
-    // 2. in-wave ranking (see radix_pass_core)
+    // 2. ranks inside the unit
+    unsigned long long* const words = s_rank + unit * 256;
+    const unsigned below_mask = (1u << l32) - 1u;
+    if (lds_rank) {
+        const unsigned long long add = (1ull << 32) | (1ull << l32);
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const unsigned d = dig[k];
-        int m[8];
-        unsigned long long bal[8];
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
-            __asm__("" : "+v"(m[b]));
-            bal[b] = ballot(m[b] != 0);
+        for (int k = 0; k < ITEMS; k++) {
+            unsigned long long* const w = words + digit_of(k);
+            __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned long long x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned mask = (unsigned)x, count = (unsigned)(x >> 32);
+            const unsigned below = (unsigned)__builtin_popcount(mask & below_mask);
+            if (below == 0u) __hip_atomic_store((unsigned*)w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            put_rank(k, count - (unsigned)__builtin_popcount(mask) + below);
+            __asm__ volatile("" ::: "memory");
         }
-        unsigned plo = ~0u, phi = ~0u;
+    } else {
+        const bool upper = lane >= 32u;
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
-            phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
+        for (int k = 0; k < ITEMS; k++) {
+            const unsigned d = digit_of(k);
+            int m[8];
+            unsigned long long bal[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
+                __asm__("" : "+v"(m[b]));
+                bal[b] = ballot(m[b] != 0);
+            }
+            unsigned plo = ~0u, phi = ~0u;
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
+                phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
+            }
+            const unsigned own = upper ? phi : plo;  // peers in this lane's unit
+            const unsigned below = (unsigned)__builtin_popcount(own & below_mask);
+            const unsigned cnt = (unsigned)__builtin_popcount(own);
+            unsigned* const ctr = (unsigned*)(words + d) + 1;
+            if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            put_rank(k, after - cnt + below);
+            if (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
         }
-        const unsigned below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
-        const unsigned cnt = (unsigned)__builtin_popcount(plo) + (unsigned)__builtin_popcount(phi);
-        unsigned* ctr = &s_wcnt[wave][d];
-        if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        dig[k] |= (after - cnt + below) << 8;
-        if (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
     }
+    // the keys look new to the compiler here, so later digits are recomputed
+    // from them rather than kept live from the ranking
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) __asm__ volatile("" : "+v"(key[k]));
     lds_barrier();
 
     // 3. counts, first lookback reads, local digit scan
@@ -744,9 +821,9 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
         }
         unsigned run = 0;
 #pragma unroll
-        for (int ww = 0; ww < WAVES; ww++) {
-            unsigned c = s_wcnt[ww][d];
-            s_wcnt[ww][d] = run;
+        for (int u = 0; u < UNITS; u++) {
+            const unsigned c = ((const unsigned*)(s_rank + u * 256 + d))[1];
+            s_off[u][d] = (unsigned short)run;
             run += c;
         }
         real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
@@ -761,47 +838,37 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
         const unsigned d = threadIdx.x;
         for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
 #pragma unroll
-        for (int ww = 0; ww < WAVES; ww++) s_wcnt[ww][d] += dstart;
+        for (int u = 0; u < UNITS; u++) s_off[u][d] = (unsigned short)(s_off[u][d] + dstart);
     }
     lds_barrier();
 
-    // 4. local positions (two 16-bit halves per register); own keys read back,
-    //    then written at their digit-sorted slot
-    unsigned lp2[ITEMS / 2];
+    // 4. local positions replace the ranks; keys into the staging buffer (the
+    //    rank words are dead)
+    unsigned* const lp2 = rk2;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const unsigned lpos = s_wcnt[wave][dig[k] & 255u] + (dig[k] >> 8);
-        if (k & 1) lp2[k / 2] |= lpos << 16;
-        else lp2[k / 2] = lpos;
-    }
-    __asm__ volatile("" ::: "memory");
-    {
-        U key[ITEMS];
+    for (int k = 0; k < ITEMS; k++) lp2[k / 2] += (unsigned)s_off[unit][digit_of(k)] << (16 * (k & 1));
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) key[k] = s_stage[wpos0 + k * 64 + lane];
-        lds_barrier();
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = key[k];
-    }
+    for (int k = 0; k < ITEMS; k++) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = key[k];
     // the value staging recomputes its addresses from the packed positions
 #pragma unroll
     for (int i = 0; i < ITEMS / 2; i++) __asm__ volatile("" : "+v"(lp2[i]));
     __asm__ volatile("" ::: "memory");
 
-    // 5. values in flight during the lookback
+    // values in flight during the lookback
     V val[VB ? ITEMS : 1];
     if (VB) {
         if (full) {
 #pragma unroll
-            for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (wpos0 + k * 64) + lane);
+            for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (upos0 + k * 32) + l32);
         } else {
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
-                unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+                unsigned p = (unsigned)(upos0 + k * 32) + l32;
                 val[k] = vin[p < valid ? p : valid - 1];
             }
         }
     }
+    // 5. finish the lookback for digit d
     if (threadIdx.x < 256) {
         const unsigned d = threadIdx.x;
         unsigned long long excl = 0;
@@ -820,7 +887,7 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
                 for (int b = 0; b < LBB; b++) {
                     if (done || used != b) break;
                     S x = ws[b];
-                    if ((x & ~SB::mask) == 0) break;
+                    if ((x & ~SB::mask) == 0) break;  // not yet published
                     excl += (unsigned long long)(x & SB::mask);
                     used = b + 1;
                     if (x & SB::incl) done = true;
@@ -835,7 +902,7 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
     }
     lds_barrier();
 
-    // 6. keys out (digits remembered, four per register), values staged and out
+    // keys out (digits remembered, four per register)
     U* kout = (U*)kout_p;
     V* vout = (V*)vout_p;
     unsigned dg4[ITEMS / 4];
@@ -852,10 +919,11 @@ TRTC_DEV void radix_pass_park(const RadixPassArgs& a)
         if (r % 8 == 7) __builtin_amdgcn_sched_barrier(0);
     }
     if (!VB) return;
+    // 6. values staged in the same buffer and written out
     lds_barrier();
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        unsigned p = (unsigned)(wpos0 + k * 64) + lane;
+        unsigned p = (unsigned)(upos0 + k * 32) + l32;
         if (full || p < valid) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = val[k];
     }
     lds_barrier();

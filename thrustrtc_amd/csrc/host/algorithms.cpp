@@ -155,26 +155,26 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     const size_t cus = (size_t)std::max(1, context().num_cus);
     const unsigned kRadixBlock = n / (1024 * (size_t)ITEMS) >= cus ? 1024u : 512u;
     // 4-byte keys with 4-byte or no values: the long-tile pass
-    // (trtc_radix_pass_park_k, TRTC_PARK_BLOCK x TRTC_PARK_ITEMS = 1024 x 24
-    // keys in the device header) for the largest sorts only.  Whole u32/u32
-    // sorts, 16384-key tiles against 24576-key tiles, one box: 2^24 0.339 vs
-    // 0.357 ms, 2^25 0.629 vs 0.661, 2^26 1.26 vs 1.25, 2^27 2.41 vs 2.38,
-    // 2^28 4.87 vs 4.64
-    constexpr size_t kParkTile = 1024 * 24;
-    const bool park = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n >= (3ull << 26) && n / kParkTile >= cus;
-    const size_t TILE = park ? kParkTile : (size_t)kRadixBlock * ITEMS;
+    // (trtc_radix_pass_long_k, TRTC_LONG_BLOCK x TRTC_LONG_ITEMS = 1024 x 32
+    // keys in the device header) for the largest sorts only; its grid is
+    // padded to whole runs of 8 tiles on each of the 8 XCDs
+    constexpr size_t kLongTile = 1024 * 32, kLongGridUnit = 64;
+    const bool long_tiles = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n >= (3ull << 26) && n / kLongTile >= cus;
+    const size_t TILE = long_tiles ? kLongTile : (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
+    const size_t grid = long_tiles ? (tiles + kLongGridUnit - 1) / kLongGridUnit * kLongGridUnit : tiles;
     const bool wide = n >= (1ull << 30);
     const size_t sw = wide ? 8 : 4;
 
     PoolBuf ktmp(n * KB, s);
     PoolBuf vtmp(val_bytes ? n * val_bytes : 0, s);
-    // scratch: hist [KB*256 u64] | done u32 (+pad) | tile counters [8 u32] | status [KB][tiles][256] |
-    // digit_start [KB*256 u64] | plan (the passes' buffers, written by the histogram pre-pass)
+    // scratch: hist [KB*256 u64] | done u32 (+pad) | tile counters [8 passes][8 u32] |
+    // status [KB][tiles][256] | digit_start [KB*256 u64] | plan (the passes' buffers and
+    // ranking modes, written by the histogram pre-pass)
     size_t off_hist = 0;
     size_t off_done = off_hist + KB * 256 * 8;
     size_t off_cnt = off_done + 16;
-    size_t off_status = align_up(off_cnt + 8 * 4, 256);
+    size_t off_status = align_up(off_cnt + 8 * 8 * 4, 256);
     size_t status_bytes_per_pass = align_up(tiles * 256 * sw, 256);
     size_t off_dstart = off_status + KB * status_bytes_per_pass;
     size_t off_plan = align_up(off_dstart + KB * 256 * 8, 256);
@@ -224,12 +224,12 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
 
     KernelRef pk;
     std::string pname =
-        park ? "trtc_radix_pass_park_k<" + std::to_string(val_bytes) + ", " +
-                   (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kind) + ">"
-             : "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
-                   (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ", " +
-                   std::to_string(kRadixBlock == 1024 ? kind : -1) + ">";
-    const unsigned pass_block = park ? 1024u : kRadixBlock;
+        long_tiles ? "trtc_radix_pass_long_k<" + std::to_string(val_bytes) + ", " +
+                         (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kind) + ">"
+                   : "trtc_radix_pass_k<" + std::to_string(KB) + ", " + std::to_string(val_bytes) + ", " +
+                         (wide ? "unsigned long long" : "unsigned int") + ", " + std::to_string(kRadixBlock) + ", " +
+                         std::to_string(kRadixBlock == 1024 ? kind : -1) + ">";
+    const unsigned pass_block = long_tiles ? 1024u : kRadixBlock;
     if (!get_template_kernel(pname, "", pk)) return false;
     trtc::RadixPassArgs pa{};
     void* kbuf[2] = {keys, ktmp.p};
@@ -242,14 +242,14 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
         pa.n = n;
         pa.digit_start = (const unsigned long long*)(sc + off_dstart) + p * 256;
         pa.status = sc + off_status + p * status_bytes_per_pass;
-        pa.tile_counter = (unsigned*)(sc + off_cnt) + p;
+        pa.tile_counter = (unsigned*)(sc + off_cnt) + p * 8;  // the long-tile pass uses all 8
         pa.shift = 8 * p;
         pa.kind = kind;
         pa.descending = descending ? 1 : 0;
         pa.plan = plan;
         pa.pass = p;
         void* args[] = {&pa};
-        if (!launch_kernel(pk, dim3((unsigned)tiles), dim3(pass_block), args, 0, s)) return false;
+        if (!launch_kernel(pk, dim3((unsigned)grid), dim3(pass_block), args, 0, s)) return false;
     }
     // the copy the plan may need (an odd number of in-place passes ran, or
     // none ran out of place); its workgroups return at once otherwise, so the
