@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 # (test_tuner_candidates_are_all_tested checks this against the tuner's own
 # candidate list)
 SCHEDULES_4 = ["rest:512x16", "rest:512x8", "rest:256x16", "<=2048:256x8 rest:512x16", "<=4096:256x16 rest:512x16",
-               "<=2048:256x8 <=262144:512x16 rest:1024x16", "<=4096:256x16 <=262144:512x16 rest:1024x16"]
+               "<=2048:256x8 <=262144:512x16 rest:1024x16", "<=4096:256x16 <=262144:512x16 rest:1024x16",
+               "<=2048:256x8 <=262144:512x16 rest:1024x32", "<=4096:256x16 <=262144:512x16 rest:1024x32"]
 SCHEDULES_8 = ["rest:512x8", "rest:256x8", "<=2048:256x8 rest:512x8", "<=2048:256x8 <=262144:512x8 rest:1024x8"]
 
 
@@ -49,9 +50,10 @@ def edge_sizes():
 
 def large_edge_sizes():
     # segments on both sides of the 2^18 class boundary of the 1024-thread
-    # schedules, and 1024x16 (16384-key) / 1024x8 (8192-key) tile edges
-    return [262143, 262144, 262145, 262144 + 8191, 262144 + 8193, 262144 + 16383, 262144 + 16385, 1 << 20,
-            (1 << 20) + 1]
+    # schedules, and 1024x32 (32768-key) / 1024x16 (16384-key) / 1024x8
+    # (8192-key) tile edges
+    return [262143, 262144, 262145, 262144 + 8191, 262144 + 8193, 262144 + 16383, 262144 + 16385,
+            262144 + 32767, 262144 + 32769, 10 * 32768, 10 * 32768 + 1, 1 << 20, (1 << 20) + 1]
 
 
 def sizes_for(schedule):
@@ -123,7 +125,8 @@ def test_segmented_sort_by_key_8byte(trtc, orc, wl, schedule, kind):
 
 
 @pytest.mark.parametrize("schedule", ["rest:512x16", "<=2048:256x8 rest:512x16",
-                                      "<=2048:256x8 <=262144:512x16 rest:1024x16"])
+                                      "<=2048:256x8 <=262144:512x16 rest:1024x16",
+                                      "<=2048:256x8 <=262144:512x16 rest:1024x32", "rest:1024x32"])
 def test_segmented_sort_keys_only(trtc, orc, wl, schedule):
     say(f"segmented keys-only sort schedule={schedule}")
     offs = offsets_of(sizes_for(schedule))
@@ -181,6 +184,13 @@ def test_segmented_sort_rejects_bad_offsets(trtc):
         trtc.Segmented_Sort_By_Key(dk, dv, [0, 101])
     with pytest.raises(ValueError):
         trtc.Set_Batch_Schedule("rest:300x16")
+    with pytest.raises(ValueError):
+        trtc.Set_Batch_Schedule("rest:512x32")  # 32 keys per thread only in the 1024-thread long tiles
+    trtc.Set_Batch_Schedule("rest:1024x32")  # long tiles: 4-byte keys with 4-byte or no values only
+    d8 = trtc.device_vector("uint64_t", 100)
+    with pytest.raises(SystemError):
+        trtc.Segmented_Sort_By_Key(d8, dv, [0, 50, 100])
+    trtc.Set_Batch_Schedule(None)
 
 
 def test_segmented_log_uniform_autotuned(trtc, orc, wl):

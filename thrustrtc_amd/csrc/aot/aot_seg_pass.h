@@ -11,10 +11,18 @@ template <int KB, int VB, int BLOCK, int ITEMS, int KIND> void reg_seg_pass()
                     std::to_string(ITEMS) + ", " + std::to_string(KIND) + ">";
     TRTC_REG(("trtc_seg_radix_pass_k<" + p).c_str(), (&trtc_seg_radix_pass_k<KB, VB, BLOCK, ITEMS, KIND>));
 }
+template <int VB, int KIND> void reg_seg_pass_long()
+{
+    std::string p = std::to_string(VB) + ", " + std::to_string(KIND) + ">";
+    TRTC_REG(("trtc_seg_radix_pass_long_k<" + p).c_str(), (&trtc_seg_radix_pass_long_k<VB, KIND>));
+}
 template <int KIND> struct RegSegPass
 {
     RegSegPass()
     {
+        // 4-byte keys with 4-byte or no values: the long-tile pass (1024 x 32)
+        reg_seg_pass_long<4, KIND>();
+        reg_seg_pass_long<0, KIND>();
         // 4-byte keys: keys only and 4-byte values
         reg_seg_pass<4, 4, 1024, 16, KIND>();
         reg_seg_pass<4, 4, 512, 16, KIND>();

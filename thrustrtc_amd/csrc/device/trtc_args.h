@@ -150,11 +150,12 @@ struct SegRadixArgs
     const unsigned* tile_seg;  // [num_tiles] segment index of each launch tile
     unsigned num_tiles;
     unsigned num_multi_tiles;  // tiles 0..num_multi_tiles-1 belong to multi-tile segments
-    unsigned* tile_counter;    // zero on entry (one per pass)
+    unsigned* tile_counter;    // zero on entry (one per pass; 8 per pass for the long-tile pass)
     unsigned* status;          // [num_multi_tiles][256] lookback words of this pass, zero on entry
     unsigned* seg_hist;        // [multi segments][KB][256] digit counts, zero on entry
     unsigned* seg_dstart;      // [multi segments][KB][256] exclusive digit offsets
     unsigned* seg_done;        // [multi segments] arrival counters, zero on entry
+    unsigned* seg_rank_lds;    // [multi segments] bit p: the long-tile pass ranks digit p with LDS words
     const unsigned* hist_chunk;  // [num_hist_chunks][2]: segment index, chunk index inside it
     unsigned num_hist_chunks;
     unsigned hist_chunk_keys;    // keys per histogram chunk

@@ -240,6 +240,14 @@ __global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArg
     trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB, KIND>(a);
 }
 
+// The long-tile pass over the large segments of a batch (1024 x 32 keys;
+// grid padded to whole runs of 8 tiles per XCD).
+template <int VB, int KIND>
+__global__ void __launch_bounds__(TRTC_LONG_BLOCK) trtc_seg_radix_pass_long_k(trtc::SegRadixArgs a)
+{
+    trtc::seg_radix_pass_long<TRTC_LONG_BLOCK, TRTC_LONG_ITEMS, VB, KIND>(a);
+}
+
 template <int KB>
 __global__ void __launch_bounds__(TRTC_HIST_BLOCK) trtc_seg_radix_hist_k(trtc::SegRadixArgs a)
 {

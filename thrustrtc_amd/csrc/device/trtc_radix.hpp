@@ -622,8 +622,26 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
     radix_pass_core<BLOCK, ITEMS, KB, VB, S, KIND, false>(a.tile_counter, make, a.shift, a.kind, a.descending);
 }
 
-// Digit pass for large single sorts of 4-byte keys with 4-byte or no values:
-// BLOCK x ITEMS = 1024 x 32 keys per tile (128-key digit runs on average, so
+// What one workgroup of the long-tile pass works on: tile `tile` of one key
+// sequence (the whole input of a sort, or one segment of a batch).
+template <class S> struct LongJob
+{
+    const unsigned* kin;
+    const unsigned* vin;
+    unsigned* kout;
+    unsigned* vout;
+    size_t n;                             // keys in the sequence
+    size_t tile;                          // tile index inside the sequence
+    S* status;                            // lookback rows of the sequence; null for a one-tile sequence
+    const unsigned long long* dstart64;   // [256] digit offsets of the pass, or
+    const unsigned* dstart32;             // [256] (null both: one-tile sequence)
+    unsigned lead;                        // keys tile 0 gives up: tile t >= 1 starts at t*TILE - lead
+    bool lds_rank;                        // rank with LDS count|mask words (else ballots)
+};
+
+// Long-tile digit pass for large sorts of 4-byte keys with 4-byte or no
+// values (single sorts, and the large segments of a batch): BLOCK x ITEMS =
+// 1024 x 32 keys per tile (128-key digit runs on average, so
 // fewer partially written 128-byte lines at run ends than the 64-key runs of
 // 16384-key tiles), ranked in 32-lane units:
 //   * unit u = threadIdx.x / 32 owns the 32*ITEMS consecutive tile positions
@@ -649,7 +667,8 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 // then meet in one L2 instead of reaching HBM separately.  2^28 u32/u32, one
 // pass, uniform keys: 1.105 ms (1024 x 24, wave ranking, arrival order) ->
 // 0.928 ms; low-entropy keys 1.080 -> 1.009 ms.
-//   1. take a tile (grid padded to whole runs; tiles past the end return),
+//   1. take a tile (grid padded to whole runs of 8 per XCD; tiles past the
+//      end return),
 //   2. load keys, rank in units,
 //   3. publish the tile's digit counts, local digit scan, first lookback reads,
 //   4. write keys to LDS at their digit-sorted slot; load the values,
@@ -657,8 +676,10 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
 //   6. stage the values in the same buffer and write them out.
 // Stable: a unit's ranks follow its positions (slot, then lane) and units are
 // scanned in position order.
-template <int BLOCK, int ITEMS, int VB, class S, int KIND>
-TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
+
+template <int BLOCK, int ITEMS, int VB, class S, int KIND, class MakeJob>
+TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, const MakeJob& make_job, int shift,
+                                   int kind, int descending)
 {
     typedef unsigned U;
     typedef unsigned V;
@@ -672,21 +693,6 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
     static_assert(VB == 0 || VB == 4, "4-byte values or none");
     static_assert(TILE * 4 >= UNITS * 256 * 8, "rank words fit the staging buffer");
     static_assert(TILE <= 65536 && ITEMS % 4 == 0, "16-bit local positions, digits packed by four");
-
-    const void* kin_p = a.keys_in;
-    const void* vin_p = a.vals_in;
-    void* kout_p = a.keys_out;
-    void* vout_p = a.vals_out;
-    bool lds_rank = false;
-    if (a.plan) {
-        const RadixPlan& PL = *a.plan;
-        if (!PL.kin[a.pass]) return;
-        kin_p = (const void*)PL.kin[a.pass];
-        vin_p = (const void*)PL.vin[a.pass];
-        kout_p = (void*)PL.kout[a.pass];
-        vout_p = (void*)PL.vout[a.pass];
-        lds_rank = (PL.rank_lds >> a.pass) & 1u;
-    }
 
     __shared__ __attribute__((aligned(16))) unsigned s_stage[TILE];
     __shared__ unsigned short s_off[UNITS][256];  // unit's first local slot per digit
@@ -703,30 +709,32 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
     }
     if (threadIdx.x == 0) {
         const unsigned x = blockIdx.x & 7u;
-        const unsigned k = atomicAdd(a.tile_counter + x, 1u);
+        const unsigned k = atomicAdd(tile_counter + x, 1u);
         s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
     }
     lds_barrier();
+    if ((size_t)s_tile >= num_tiles) return;  // padding of the grid
 
-    const size_t tile = s_tile;
-    const size_t n = a.n;
-    const size_t tile_base = tile * (size_t)TILE;
-    if (tile_base >= n) return;  // padding of the grid
+    const LongJob<S> job = make_job((unsigned)s_tile);
+    const size_t tile = job.tile;
+    const size_t n = job.n;
+    // tile 0 holds TILE - lead keys, so that with lead = the sequence's
+    // offset from a 128-byte line in keys every later tile starts on a line
+    const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
+    const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
+    const unsigned valid = (unsigned)(tile_end - tile_base);
+    const bool full = valid == (unsigned)TILE;
     const unsigned wave = threadIdx.x / 64;
     const unsigned lane = lane_id();
     const unsigned l32 = lane & 31u;
     const unsigned unit = threadIdx.x >> 5;
-    const bool full = tile_base + TILE <= n;
-    const unsigned valid = full ? (unsigned)TILE : (unsigned)(n - tile_base);
-    const U* kin = (const U*)kin_p + tile_base;
-    const V* vin = (const V*)vin_p + tile_base;
+    const U* kin = job.kin + tile_base;
+    const V* vin = job.vin + tile_base;
     const int upos0 = (int)unit * UITEMS;
-    const int shift = a.shift;
-    const int kind = a.kind;
-    const U flip = a.descending ? ~0u : 0u;
-    S* st = (S*)a.status;
+    const U flip = descending ? ~0u : 0u;
+    S* st = job.status;
     const bool head = tile == 0;
-
+    const bool lds_rank = job.lds_rank;
     // 1. keys (registers)
     U key[ITEMS];
     unsigned in_range = ~0u;  // bit k: item k lies inside the input
@@ -827,7 +835,7 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
             run += c;
         }
         real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
-        st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+        if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
         Plus plus;
         unsigned inc = wave_inclusive_scan(run, plus);
         if (lane == 63) s_dscan[wave] = inc;
@@ -898,13 +906,17 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
             }
             st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
         }
-        s_base[d] = (P)(a.digit_start[d] + excl - dstart);
+        // digit offset of the sequence: from the histogram pre-pass, or for a
+        // sequence of one tile, the tile's own digit scan
+        const unsigned long long ds = job.dstart64 ? job.dstart64[d]
+                                    : (job.dstart32 ? (unsigned long long)job.dstart32[d] : (unsigned long long)dstart);
+        s_base[d] = (P)(ds + excl - dstart);
     }
     lds_barrier();
 
     // keys out (digits remembered, four per register)
-    U* kout = (U*)kout_p;
-    V* vout = (V*)vout_p;
+    U* kout = job.kout;
+    V* vout = job.vout;
     unsigned dg4[ITEMS / 4];
 #pragma unroll
     for (int r = 0; r < ITEMS; r++) {
@@ -940,6 +952,46 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
         const P gpos = s_base[d] + j;
         if ((full || j < valid) && gpos < (P)n) vout[gpos] = vb;
     }
+}
+
+
+// One long-tile digit pass over the whole input of a sort (buffers and
+// ranking mode from the plan; a pass over a constant digit returns at once).
+template <int BLOCK, int ITEMS, int VB, class S, int KIND>
+TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
+{
+    const void* kin = a.keys_in;
+    const void* vin = a.vals_in;
+    void* kout = a.keys_out;
+    void* vout = a.vals_out;
+    bool lds_rank = false;
+    if (a.plan) {
+        const RadixPlan& PL = *a.plan;
+        if (!PL.kin[a.pass]) return;
+        kin = (const void*)PL.kin[a.pass];
+        vin = (const void*)PL.vin[a.pass];
+        kout = (void*)PL.kout[a.pass];
+        vout = (void*)PL.vout[a.pass];
+        lds_rank = (PL.rank_lds >> a.pass) & 1u;
+    }
+    constexpr size_t TILE = (size_t)BLOCK * ITEMS;
+    auto make = [&](unsigned t) {
+        LongJob<S> j;
+        j.kin = (const unsigned*)kin;
+        j.vin = (const unsigned*)vin;
+        j.kout = (unsigned*)kout;
+        j.vout = (unsigned*)vout;
+        j.n = a.n;
+        j.tile = t;
+        j.status = (S*)a.status;
+        j.dstart64 = a.digit_start;
+        j.dstart32 = nullptr;
+        j.lead = 0;
+        j.lds_rank = lds_rank;
+        return j;
+    };
+    radix_pass_long_core<BLOCK, ITEMS, VB, S, KIND>(a.tile_counter, (a.n + TILE - 1) / TILE, make, a.shift, a.kind,
+                                                     a.descending);
 }
 
 // The copy a plan may ask for after the passes (the result of an odd number
@@ -1012,6 +1064,42 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
                                                                 a.descending);
 }
 
+// The long-tile pass (radix_pass_long_core) over the segments of a batch
+// class: the tile map gives the segment, lookback stops at its first tile,
+// digit offsets and the ranking mode come from the segment histograms; buffers
+// as seg_radix_pass.  One-tile segments sort in place with their own digit
+// scan and ballot ranking.
+template <int BLOCK, int ITEMS, int VB, int KIND>
+TRTC_DEV void seg_radix_pass_long(const SegRadixArgs& a)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    auto make = [&](unsigned t) {
+        const SegSortDesc& d = a.descs[a.tile_seg[t]];
+        const bool odd = a.pass & 1;
+        LongJob<unsigned> j;
+        j.kin = (const unsigned*)(a.pass == 0 ? d.src_keys : (odd ? d.tmp_keys : d.keys));
+        j.vin = (const unsigned*)(a.pass == 0 ? d.src_vals : (odd ? d.tmp_vals : d.vals));
+        j.kout = (unsigned*)(odd ? d.keys : d.tmp_keys);
+        j.vout = (unsigned*)(odd ? d.vals : d.tmp_vals);
+        j.n = d.n;
+        j.tile = t - d.first_tile;
+        j.lead = d.lead;
+        j.dstart64 = nullptr;
+        if (d.n > (unsigned long long)TILE) {
+            j.status = a.status + d.status_row * 256;
+            j.dstart32 = a.seg_dstart + ((size_t)d.multi * 4 + a.pass) * 256;
+            j.lds_rank = (a.seg_rank_lds[d.multi] >> a.pass) & 1u;
+        } else {
+            j.status = nullptr;
+            j.dstart32 = nullptr;
+            j.lds_rank = false;
+        }
+        return j;
+    };
+    radix_pass_long_core<BLOCK, ITEMS, VB, unsigned, KIND>(a.tile_counter, a.num_tiles, make, a.shift, a.kind,
+                                                            a.descending);
+}
+
 // Digit histograms of the multi-tile segments of a batch: one workgroup per
 // chunk of hist_chunk_keys keys (all KB digits from one read, bank-conflict
 // free LDS counters as in radix_hist, one global add per bin and workgroup); the last chunk of a
@@ -1057,18 +1145,29 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     __syncthreads();
     if (!s_last) return;
     unsigned* ds = a.seg_dstart + (size_t)d.multi * KB * 256;
+    // digit offsets, and the ranking mode of the long-tile pass per digit (as
+    // radix_hist: LDS rank words when sum (h_d / n)^2 < 1/32)
     if (threadIdx.x < 64) {
+        unsigned rank_lds = 0;
+        const double inv_n = 1.0 / (double)d.n;
         for (int p = 0; p < KB; p++) {
             unsigned run = 0;
+            double sq = 0.0;
             for (int q = 0; q < 4; q++) {
                 int dd = q * 64 + (int)threadIdx.x;
                 unsigned x = ld_relaxed(hist + p * 256 + dd);
+                const double f = (double)x * inv_n;
+                sq += f * f;
                 Plus plus;
                 unsigned inc = wave_inclusive_scan(x, plus);
                 ds[p * 256 + dd] = run + inc - x;
                 run += shfl(inc, 63);
             }
+            Plus plus;
+            sq = shfl(wave_inclusive_scan(sq, plus), 63);
+            if (sq < 1.0 / 32) rank_lds |= 1u << p;
         }
+        if (threadIdx.x == 0 && a.seg_rank_lds) a.seg_rank_lds[d.multi] = rank_lds;
     }
 }
 

@@ -63,13 +63,22 @@ SortSchedule default_sort_schedule(int key_bytes, int val_bytes)
     return {{kNoLimit, 512, 16}};
 }
 
+// The long-tile pass (1024 x 32 keys, trtc_seg_radix_pass_long_k) exists for
+// 4-byte keys with 4-byte or no values.
+static bool long_tiles_ok(int key_bytes, int val_bytes) { return key_bytes == 4 && (val_bytes == 0 || val_bytes == 4); }
+static bool is_long_class(int key_bytes, int val_bytes, int block, int items)
+{
+    return long_tiles_ok(key_bytes, val_bytes) && block == 1024 && items == 32;
+}
+
 // Size classes for batches whose shape changes from call to call (the request
 // server): small segments in small tiles, segments above 2^18 keys in the
-// 1024-thread tiles of large single sorts (the autotuner's choice on
-// log-uniform sizes, e.g. the 4096-segment batch).
+// tiles of large single sorts (the autotuner's choice on log-uniform sizes,
+// e.g. the 4096-segment batch).
 SortSchedule served_sort_schedule(int key_bytes, int val_bytes)
 {
     if (key_bytes > 4 || val_bytes > 4) return {{2048, 256, 8}, {1u << 18, 512, 8}, {kNoLimit, 1024, 8}};
+    if (long_tiles_ok(key_bytes, val_bytes)) return {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 32}};
     return {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}};
 }
 
@@ -108,6 +117,11 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
     plan.desc = descending;
     plan.cls.resize(sched.size());
     for (size_t c = 0; c < sched.size(); c++) {
+        if (sched[c].items == 32 && !is_long_class(KB, VB, sched[c].block, sched[c].items)) {
+            TRTC_ERR("batched sort: %dx%d tiles are for 4-byte keys with 4-byte or no values (1024 x 32)",
+                     sched[c].block, sched[c].items);
+            return false;
+        }
         plan.cls[c].block = sched[c].block;
         plan.cls[c].items = sched[c].items;
         plan.cls[c].tile = (size_t)sched[c].block * sched[c].items;
@@ -186,6 +200,8 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
         C.zero_bytes = off - C.off_zero;
         C.off_dstart = off;
         off += align256((size_t)C.multi_segs * KB * 256 * 4);
+        C.off_rank = off;  // written by the histogram pre-pass for every multi-tile segment
+        off += align256((size_t)C.multi_segs * 4);
     }
     plan.scratch_bytes = off;
     return true;
@@ -240,6 +256,8 @@ namespace {
 std::string seg_hist_name(const SortPlan& plan) { return "trtc_seg_radix_hist_k<" + std::to_string(plan.KB) + ">"; }
 std::string seg_pass_name(const SortPlan& plan, const SortPlan::Cls& C)
 {
+    if (is_long_class(plan.KB, plan.VB, C.block, C.items))
+        return "trtc_seg_radix_pass_long_k<" + std::to_string(plan.VB) + ", " + std::to_string(plan.kind) + ">";
     return "trtc_seg_radix_pass_k<" + std::to_string(plan.KB) + ", " + std::to_string(plan.VB) + ", " +
            std::to_string(C.block) + ", " + std::to_string(C.items) + ", " + std::to_string(plan.kind) + ">";
 }
@@ -288,6 +306,7 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
         a.seg_hist = (unsigned*)(scratch + C.off_hist);
         a.seg_dstart = (unsigned*)(scratch + C.off_dstart);
         a.seg_done = (unsigned*)(scratch + C.off_done);
+        a.seg_rank_lds = (unsigned*)(scratch + C.off_rank);
         a.hist_chunk = (const unsigned*)(scratch + C.off_hist_chunk);
         a.num_hist_chunks = C.hist_chunks;
         a.hist_chunk_keys = (unsigned)plan.hist_chunk;
@@ -303,13 +322,17 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
         }
         KernelRef pk;
         if (!get_template_kernel(seg_pass_name(plan, C), "", pk)) return false;
+        // the long-tile pass hands tiles to the 8 XCDs in runs of 8 (a grid of
+        // whole runs, 8 tile counters per pass)
+        const bool lt = is_long_class(plan.KB, plan.VB, C.block, C.items);
+        const unsigned grid = lt ? (C.num_tiles + 63u) / 64u * 64u : C.num_tiles;
         for (int p = 0; p < plan.KB; p++) {
             a.pass = p;
             a.shift = 8 * p;
-            a.tile_counter = (unsigned*)(scratch + C.off_cnt) + p;
+            a.tile_counter = (unsigned*)(scratch + C.off_cnt) + (lt ? 8 * p : p);
             a.status = (unsigned*)(scratch + C.off_status) + p * C.status_pass_words;
             void* args[] = {&a};
-            if (!launch_kernel(pk, dim3(C.num_tiles), dim3(C.block), args, 0, s)) return false;
+            if (!launch_kernel(pk, dim3(grid), dim3(C.block), args, 0, s)) return false;
         }
     }
     return true;
@@ -397,8 +420,8 @@ std::string signature(const std::vector<SortSeg>& segs, int KB, int VB)
 }
 
 // Segments above 2^18 keys may take 1024-thread tiles (16384 keys for 4-byte
-// pairs, as the single large sort uses); the tuner keeps them only where
-// they win.
+// pairs, or the 32768-key long tiles of the largest single sorts for 4-byte
+// keys with 4-byte or no values); the tuner keeps them only where they win.
 std::vector<SortSchedule> candidates(int KB, int VB)
 {
     if (KB > 4 || VB > 4)
@@ -406,13 +429,18 @@ std::vector<SortSchedule> candidates(int KB, int VB)
                 {{kNoLimit, 256, 8}},
                 {{2048, 256, 8}, {kNoLimit, 512, 8}},
                 {{2048, 256, 8}, {1u << 18, 512, 8}, {kNoLimit, 1024, 8}}};
-    return {{{kNoLimit, 512, 16}},
-            {{kNoLimit, 512, 8}},
-            {{kNoLimit, 256, 16}},
-            {{2048, 256, 8}, {kNoLimit, 512, 16}},
-            {{4096, 256, 16}, {kNoLimit, 512, 16}},
-            {{2048, 256, 8}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}},
-            {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}}};
+    std::vector<SortSchedule> c = {{{kNoLimit, 512, 16}},
+                                   {{kNoLimit, 512, 8}},
+                                   {{kNoLimit, 256, 16}},
+                                   {{2048, 256, 8}, {kNoLimit, 512, 16}},
+                                   {{4096, 256, 16}, {kNoLimit, 512, 16}},
+                                   {{2048, 256, 8}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}},
+                                   {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}}};
+    if (long_tiles_ok(KB, VB)) {
+        c.push_back({{2048, 256, 8}, {1u << 18, 512, 16}, {kNoLimit, 1024, 32}});
+        c.push_back({{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 32}});
+    }
+    return c;
 }
 
 // Copies the segments' data into a shadow buffer (adjacent segments in one
@@ -463,7 +491,8 @@ bool parse_schedule(const std::string& spec, SortSchedule& out)
         } else {
             return false;
         }
-        if (!(b == 256 || b == 512 || b == 1024) || !(it == 4 || it == 8 || it == 16)) return false;
+        if (!(b == 256 || b == 512 || b == 1024) || !(it == 4 || it == 8 || it == 16 || (it == 32 && b == 1024)))
+            return false;
         c.block = b;
         c.items = it;
         out.push_back(c);

@@ -46,7 +46,7 @@ struct SortPlan
         size_t off_descs = 0, off_tile_seg = 0, off_hist_chunk = 0;  // in the uploaded table block
         size_t off_zero = 0, zero_bytes = 0;         // in scratch: cleared before the launches
         size_t off_cnt = 0, off_done = 0, off_hist = 0, off_status = 0, status_pass_words = 0;
-        size_t off_dstart = 0;
+        size_t off_dstart = 0, off_rank = 0;
     };
     std::vector<Cls> cls;
     size_t hist_chunk = 131072;  // keys per histogram workgroup
