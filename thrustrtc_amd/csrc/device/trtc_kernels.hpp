@@ -219,9 +219,9 @@ __global__ void __launch_bounds__(BLOCK) trtc_radix_pass_k(trtc::RadixPassArgs a
 }
 
 // Long-tile digit pass (1024 x 32 keys, ranked in 32-lane units) for large
-// sorts of 4-byte keys with 4-byte or no values (trtc::radix_pass_long).  The
-// grid is padded to whole runs of 8 tiles per XCD (64 tiles); the padding
-// workgroups return at once.
+// sorts of 4-byte keys with 4-byte or no values (trtc::radix_pass_long).  Its
+// workgroups take tiles until none is left; the grid must be a multiple of 8
+// (tile runs go to virtual XCD blockIdx.x mod 8), normally one per CU.
 #define TRTC_LONG_BLOCK 1024
 #define TRTC_LONG_ITEMS 32
 template <int VB, class S, int KIND>
@@ -240,8 +240,8 @@ __global__ void __launch_bounds__(BLOCK) trtc_seg_radix_pass_k(trtc::SegRadixArg
     trtc::seg_radix_pass<BLOCK, ITEMS, KB, VB, KIND>(a);
 }
 
-// The long-tile pass over the large segments of a batch (1024 x 32 keys;
-// grid padded to whole runs of 8 tiles per XCD).
+// The long-tile pass over the large segments of a batch (1024 x 32 keys; grid
+// a multiple of 8, as trtc_radix_pass_long_k).
 template <int VB, int KIND>
 __global__ void __launch_bounds__(TRTC_LONG_BLOCK) trtc_seg_radix_pass_long_k(trtc::SegRadixArgs a)
 {

@@ -667,8 +667,8 @@ template <class S> struct LongJob
 // then meet in one L2 instead of reaching HBM separately.  2^28 u32/u32, one
 // pass, uniform keys: 1.105 ms (1024 x 24, wave ranking, arrival order) ->
 // 0.928 ms; low-entropy keys 1.080 -> 1.009 ms.
-//   1. take a tile (grid padded to whole runs of 8 per XCD; tiles past the
-//      end return),
+//   1. take a tile (the workgroup loops until no tile is left; the grid is a
+//      multiple of 8 so that every virtual XCD has workgroups),
 //   2. load keys, rank in units,
 //   3. publish the tile's digit counts, local digit scan, first lookback reads,
 //   4. write keys to LDS at their digit-sorted slot; load the values,
@@ -701,259 +701,276 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
     __shared__ unsigned s_tile;
     unsigned long long* const s_rank = (unsigned long long*)s_stage;
 
-    {
-        typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        v4u* z = (v4u*)s_rank;
-        const v4u zero = {0u, 0u, 0u, 0u};
-        for (int i = threadIdx.x; i < UNITS * 256 / 2; i += BLOCK) z[i] = zero;
-    }
-    if (threadIdx.x == 0) {
-        const unsigned x = blockIdx.x & 7u;
-        const unsigned k = atomicAdd(tile_counter + x, 1u);
-        s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
-    }
-    lds_barrier();
-    if ((size_t)s_tile >= num_tiles) return;  // padding of the grid
-
-    const LongJob<S> job = make_job((unsigned)s_tile);
-    const size_t tile = job.tile;
-    const size_t n = job.n;
-    // tile 0 holds TILE - lead keys, so that with lead = the sequence's
-    // offset from a 128-byte line in keys every later tile starts on a line
-    const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
-    const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
-    const unsigned valid = (unsigned)(tile_end - tile_base);
-    const bool full = valid == (unsigned)TILE;
-    const unsigned wave = threadIdx.x / 64;
-    const unsigned lane = lane_id();
-    const unsigned l32 = lane & 31u;
-    const unsigned unit = threadIdx.x >> 5;
-    const U* kin = job.kin + tile_base;
-    const V* vin = job.vin + tile_base;
-    const int upos0 = (int)unit * UITEMS;
-    const U flip = descending ? ~0u : 0u;
-    S* st = job.status;
-    const bool head = tile == 0;
-    const bool lds_rank = job.lds_rank;
-    // 1. keys (registers)
-    U key[ITEMS];
-    unsigned in_range = ~0u;  // bit k: item k lies inside the input
-    if (full) {
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (upos0 + k * 32) + l32);
-    } else {
-        in_range = 0;
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            unsigned p = (unsigned)(upos0 + k * 32) + l32;
-            key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
-            in_range |= (p < valid ? 1u : 0u) << k;
+    // Tiles until none is left: a grid of one workgroup per CU loops over them
+    // (no workgroup launch between tiles), a grid of one per tile takes one
+    // each.  The thread index is made opaque per tile so that per-item
+    // addresses are recomputed instead of held across the loop.
+    for (;;) {
+        unsigned tid = threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
+        {
+            typedef unsigned v4u __attribute__((ext_vector_type(4)));
+            v4u* z = (v4u*)s_rank;
+            const v4u zero = {0u, 0u, 0u, 0u};
+            for (int i = tid; i < UNITS * 256 / 2; i += BLOCK) z[i] = zero;
         }
-    }
-    // digit of item k, recomputed from the key wherever it is needed
-    // (positions past the end rank as digit 255 and are never written)
-    auto digit_of = [&](int k) -> unsigned {
-        U sk = sort_key_k<KIND, U>(key[k], kind, flip);
-        if (!full && !((in_range >> k) & 1u)) sk = ~0u;
-        return (sk >> shift) & 255u;
-    };
-    unsigned rk2[ITEMS / 2];  // ranks inside the unit, two 16-bit halves per register
-    // the asm keeps each rank's arithmetic inside its slot rather than holding
-    // every slot's LDS words until the end of the loop
-    auto put_rank = [&](int k, unsigned r) {
-        if (k & 1) rk2[k / 2] |= r << 16;
-        else rk2[k / 2] = r;
-        __asm__ volatile("" : "+v"(rk2[k / 2]));
-    };
-
-    // 2. ranks inside the unit
-    unsigned long long* const words = s_rank + unit * 256;
-    const unsigned below_mask = (1u << l32) - 1u;
-    if (lds_rank) {
-        const unsigned long long add = (1ull << 32) | (1ull << l32);
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            unsigned long long* const w = words + digit_of(k);
-            __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const unsigned long long x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const unsigned mask = (unsigned)x, count = (unsigned)(x >> 32);
-            const unsigned below = (unsigned)__builtin_popcount(mask & below_mask);
-            if (below == 0u) __hip_atomic_store((unsigned*)w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            put_rank(k, count - (unsigned)__builtin_popcount(mask) + below);
-            __asm__ volatile("" ::: "memory");
+        if (tid == 0) {
+            const unsigned x = blockIdx.x & 7u;
+            const unsigned k = atomicAdd(tile_counter + x, 1u);
+            s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
         }
-    } else {
-        const bool upper = lane >= 32u;
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            const unsigned d = digit_of(k);
-            int m[8];
-            unsigned long long bal[8];
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
-                __asm__("" : "+v"(m[b]));
-                bal[b] = ballot(m[b] != 0);
-            }
-            unsigned plo = ~0u, phi = ~0u;
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
-                phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
-            }
-            const unsigned own = upper ? phi : plo;  // peers in this lane's unit
-            const unsigned below = (unsigned)__builtin_popcount(own & below_mask);
-            const unsigned cnt = (unsigned)__builtin_popcount(own);
-            unsigned* const ctr = (unsigned*)(words + d) + 1;
-            if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            put_rank(k, after - cnt + below);
-            if (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-    // the keys look new to the compiler here, so later digits are recomputed
-    // from them rather than kept live from the ranking
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) __asm__ volatile("" : "+v"(key[k]));
-    lds_barrier();
+        lds_barrier();
+        if ((size_t)s_tile >= num_tiles) break;  // no tiles left (or padding of the grid)
 
-    // 3. counts, first lookback reads, local digit scan
-    S w[LBB];
-    unsigned dstart = 0, real = 0;
-    if (threadIdx.x < 256) {
-        const unsigned d = threadIdx.x;
-#pragma unroll
-        for (int b = 0; b < LBB; b++) {
-            long long j = (long long)tile - 1 - b;
-            w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + d) : (S)0;
-        }
-        unsigned run = 0;
-#pragma unroll
-        for (int u = 0; u < UNITS; u++) {
-            const unsigned c = ((const unsigned*)(s_rank + u * 256 + d))[1];
-            s_off[u][d] = (unsigned short)run;
-            run += c;
-        }
-        real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
-        if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
-        Plus plus;
-        unsigned inc = wave_inclusive_scan(run, plus);
-        if (lane == 63) s_dscan[wave] = inc;
-        dstart = inc - run;
-    }
-    lds_barrier();
-    if (threadIdx.x < 256) {
-        const unsigned d = threadIdx.x;
-        for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
-#pragma unroll
-        for (int u = 0; u < UNITS; u++) s_off[u][d] = (unsigned short)(s_off[u][d] + dstart);
-    }
-    lds_barrier();
-
-    // 4. local positions replace the ranks; keys into the staging buffer (the
-    //    rank words are dead)
-    unsigned* const lp2 = rk2;
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) lp2[k / 2] += (unsigned)s_off[unit][digit_of(k)] << (16 * (k & 1));
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = key[k];
-    // the value staging recomputes its addresses from the packed positions
-#pragma unroll
-    for (int i = 0; i < ITEMS / 2; i++) __asm__ volatile("" : "+v"(lp2[i]));
-    __asm__ volatile("" ::: "memory");
-
-    // values in flight during the lookback
-    V val[VB ? ITEMS : 1];
-    if (VB) {
+        const LongJob<S> job = make_job((unsigned)s_tile);
+        const size_t tile = job.tile;
+        const size_t n = job.n;
+        // tile 0 holds TILE - lead keys, so that with lead = the sequence's
+        // offset from a 128-byte line in keys every later tile starts on a line
+        const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
+        const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
+        const unsigned valid = (unsigned)(tile_end - tile_base);
+        const bool full = valid == (unsigned)TILE;
+        const unsigned wave = tid / 64;
+        const unsigned lane = lane_id();
+        const unsigned l32 = lane & 31u;
+        const unsigned unit = tid >> 5;
+        const U* kin = job.kin + tile_base;
+        const V* vin = job.vin + tile_base;
+        const int upos0 = (int)unit * UITEMS;
+        const U flip = descending ? ~0u : 0u;
+        S* st = job.status;
+        const bool head = tile == 0;
+        const bool lds_rank = job.lds_rank;
+        // 1. keys (registers)
+        U key[ITEMS];
+        unsigned in_range = ~0u;  // bit k: item k lies inside the input
         if (full) {
-#pragma unroll
-            for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (upos0 + k * 32) + l32);
+    #pragma unroll
+            for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (upos0 + k * 32) + l32);
         } else {
-#pragma unroll
+            in_range = 0;
+    #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
                 unsigned p = (unsigned)(upos0 + k * 32) + l32;
-                val[k] = vin[p < valid ? p : valid - 1];
+                key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
+                in_range |= (p < valid ? 1u : 0u) << k;
             }
         }
-    }
-    // 5. finish the lookback for digit d
-    if (threadIdx.x < 256) {
-        const unsigned d = threadIdx.x;
-        unsigned long long excl = 0;
-        if (!head) {
-            long long j = (long long)tile - 1;
-            bool first = true;
-            while (true) {
-                S ws[LBB];
-#pragma unroll
-                for (int b = 0; b < LBB; b++)
-                    ws[b] = first ? w[b] : ((j - b >= 0) ? ld_relaxed(st + (size_t)(j - b) * 256 + d) : (S)0);
-                first = false;
-                bool done = false;
-                int used = 0;
-#pragma unroll
-                for (int b = 0; b < LBB; b++) {
-                    if (done || used != b) break;
-                    S x = ws[b];
-                    if ((x & ~SB::mask) == 0) break;  // not yet published
-                    excl += (unsigned long long)(x & SB::mask);
-                    used = b + 1;
-                    if (x & SB::incl) done = true;
-                }
-                if (done) break;
-                if (used == 0) __builtin_amdgcn_s_sleep(1);
-                j -= used;
-            }
-            st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
-        }
-        // digit offset of the sequence: from the histogram pre-pass, or for a
-        // sequence of one tile, the tile's own digit scan
-        const unsigned long long ds = job.dstart64 ? job.dstart64[d]
-                                    : (job.dstart32 ? (unsigned long long)job.dstart32[d] : (unsigned long long)dstart);
-        s_base[d] = (P)(ds + excl - dstart);
-    }
-    lds_barrier();
+        // digit of item k, recomputed from the key wherever it is needed
+        // (positions past the end rank as digit 255 and are never written)
+        auto digit_of = [&](int k) -> unsigned {
+            U sk = sort_key_k<KIND, U>(key[k], kind, flip);
+            if (!full && !((in_range >> k) & 1u)) sk = ~0u;
+            return (sk >> shift) & 255u;
+        };
+        unsigned rk2[ITEMS / 2];  // ranks inside the unit, two 16-bit halves per register
+        // the asm keeps each rank's arithmetic inside its slot rather than holding
+        // every slot's LDS words until the end of the loop
+        auto put_rank = [&](int k, unsigned r) {
+            if (k & 1) rk2[k / 2] |= r << 16;
+            else rk2[k / 2] = r;
+            __asm__ volatile("" : "+v"(rk2[k / 2]));
+        };
 
-    // keys out (digits remembered, four per register)
-    U* kout = job.kout;
-    V* vout = job.vout;
-    unsigned dg4[ITEMS / 4];
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const unsigned j = (unsigned)(r * BLOCK) + threadIdx.x;
-        const U kb = s_stage[j];
-        const unsigned d = (sort_key_k<KIND, U>(kb, kind, flip) >> shift) & 255u;
-        if (r % 4 == 0) dg4[r / 4] = d;
-        else dg4[r / 4] |= d << (8 * (r % 4));
-        const P gpos = s_base[d] + j;
-        // gpos < n by construction; the check keeps a logic error from writing outside the output
-        if ((full || j < valid) && gpos < (P)n) kout[gpos] = kb;
-        if (r % 8 == 7) __builtin_amdgcn_sched_barrier(0);
-    }
-    if (!VB) return;
-    // 6. values staged in the same buffer and written out
-    lds_barrier();
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        unsigned p = (unsigned)(upos0 + k * 32) + l32;
-        if (full || p < valid) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = val[k];
-    }
-    lds_barrier();
-    // slot indices and digit addresses recomputed here, not kept live from the key write-out
-    unsigned tid2 = threadIdx.x;
-    __asm__ volatile("" : "+v"(tid2));
-#pragma unroll
-    for (int i = 0; i < ITEMS / 4; i++) __asm__ volatile("" : "+v"(dg4[i]));
-#pragma unroll
-    for (int r = 0; r < ITEMS; r++) {
-        const unsigned j = (unsigned)(r * BLOCK) + tid2;
-        const V vb = s_stage[j];
-        const unsigned d = (dg4[r / 4] >> (8 * (r % 4))) & 255u;
-        const P gpos = s_base[d] + j;
-        if ((full || j < valid) && gpos < (P)n) vout[gpos] = vb;
+        // 2. ranks inside the unit
+        unsigned long long* const words = s_rank + unit * 256;
+        const unsigned below_mask = (1u << l32) - 1u;
+        if (lds_rank) {
+            const unsigned long long add = (1ull << 32) | (1ull << l32);
+    #pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                unsigned long long* const w = words + digit_of(k);
+                __hip_atomic_fetch_add(w, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const unsigned long long x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const unsigned mask = (unsigned)x, count = (unsigned)(x >> 32);
+                const unsigned below = (unsigned)__builtin_popcount(mask & below_mask);
+                if (below == 0u) __hip_atomic_store((unsigned*)w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                put_rank(k, count - (unsigned)__builtin_popcount(mask) + below);
+                __asm__ volatile("" ::: "memory");
+            }
+        } else {
+            const bool upper = lane >= 32u;
+    #pragma unroll
+            for (int k = 0; k < ITEMS; k++) {
+                const unsigned d = digit_of(k);
+                int m[8];
+                unsigned long long bal[8];
+    #pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    m[b] = __builtin_amdgcn_sbfe((int)d, b, 1);
+                    __asm__("" : "+v"(m[b]));
+                    bal[b] = ballot(m[b] != 0);
+                }
+                unsigned plo = ~0u, phi = ~0u;
+    #pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    plo = __builtin_amdgcn_bitop3_b32(plo, (unsigned)bal[b], (unsigned)m[b], 0x90);
+                    phi = __builtin_amdgcn_bitop3_b32(phi, (unsigned)(bal[b] >> 32), (unsigned)m[b], 0x90);
+                }
+                const unsigned own = upper ? phi : plo;  // peers in this lane's unit
+                const unsigned below = (unsigned)__builtin_popcount(own & below_mask);
+                const unsigned cnt = (unsigned)__builtin_popcount(own);
+                unsigned* const ctr = (unsigned*)(words + d) + 1;
+                if (below == 0u) __hip_atomic_fetch_add(ctr, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const unsigned after = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                put_rank(k, after - cnt + below);
+                if (k % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // the keys look new to the compiler here, so later digits are recomputed
+        // from them rather than kept live from the ranking
+    #pragma unroll
+        for (int k = 0; k < ITEMS; k++) __asm__ volatile("" : "+v"(key[k]));
+        lds_barrier();
+
+        // 3. counts, first lookback reads, local digit scan
+        S w[LBB];
+        unsigned dstart = 0, real = 0;
+        if (tid < 256) {
+            const unsigned d = tid;
+    #pragma unroll
+            for (int b = 0; b < LBB; b++) {
+                long long j = (long long)tile - 1 - b;
+                w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + d) : (S)0;
+            }
+            unsigned run = 0;
+    #pragma unroll
+            for (int u = 0; u < UNITS; u++) {
+                const unsigned c = ((const unsigned*)(s_rank + u * 256 + d))[1];
+                s_off[u][d] = (unsigned short)run;
+                run += c;
+            }
+            real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
+            if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+            Plus plus;
+            unsigned inc = wave_inclusive_scan(run, plus);
+            if (lane == 63) s_dscan[wave] = inc;
+            dstart = inc - run;
+        }
+        lds_barrier();
+        if (tid < 256) {
+            const unsigned d = tid;
+            for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
+    #pragma unroll
+            for (int u = 0; u < UNITS; u++) s_off[u][d] = (unsigned short)(s_off[u][d] + dstart);
+        }
+        lds_barrier();
+
+        // 4. local positions replace the ranks; keys into the staging buffer (the
+        //    rank words are dead)
+        unsigned* const lp2 = rk2;
+    #pragma unroll
+        for (int k = 0; k < ITEMS; k++) lp2[k / 2] += (unsigned)s_off[unit][digit_of(k)] << (16 * (k & 1));
+    #pragma unroll
+        for (int k = 0; k < ITEMS; k++) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = key[k];
+        // the value staging recomputes its addresses from the packed positions
+    #pragma unroll
+        for (int i = 0; i < ITEMS / 2; i++) __asm__ volatile("" : "+v"(lp2[i]));
+        __asm__ volatile("" ::: "memory");
+
+        // values in flight during the lookback (the load positions recomputed, not
+        // kept live from the key loads)
+        V val[VB ? ITEMS : 1];
+        if (VB) {
+            unsigned vpos0 = (unsigned)upos0 + l32;
+            __asm__ volatile("" : "+v"(vpos0));
+            if (full) {
+    #pragma unroll
+                for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (vpos0 + k * 32));
+            } else {
+    #pragma unroll
+                for (int k = 0; k < ITEMS; k++) {
+                    unsigned p = vpos0 + (unsigned)(k * 32);
+                    val[k] = vin[p < valid ? p : valid - 1];
+                }
+            }
+        }
+        // 5. finish the lookback for digit d
+        if (tid < 256) {
+            const unsigned d = tid;
+            unsigned long long excl = 0;
+            if (!head) {
+                long long j = (long long)tile - 1;
+                bool first = true;
+                while (true) {
+                    S ws[LBB];
+    #pragma unroll
+                    for (int b = 0; b < LBB; b++)
+                        ws[b] = first ? w[b] : ((j - b >= 0) ? ld_relaxed(st + (size_t)(j - b) * 256 + d) : (S)0);
+                    first = false;
+                    bool done = false;
+                    int used = 0;
+    #pragma unroll
+                    for (int b = 0; b < LBB; b++) {
+                        if (done || used != b) break;
+                        S x = ws[b];
+                        if ((x & ~SB::mask) == 0) break;  // not yet published
+                        excl += (unsigned long long)(x & SB::mask);
+                        used = b + 1;
+                        if (x & SB::incl) done = true;
+                    }
+                    if (done) break;
+                    if (used == 0) __builtin_amdgcn_s_sleep(1);
+                    j -= used;
+                }
+                st_relaxed(st + tile * 256 + d, (S)(SB::incl | (S)(excl + real)));
+            }
+            // digit offset of the sequence: from the histogram pre-pass, or for a
+            // sequence of one tile, the tile's own digit scan
+            const unsigned long long ds = job.dstart64 ? job.dstart64[d]
+                                        : (job.dstart32 ? (unsigned long long)job.dstart32[d] : (unsigned long long)dstart);
+            s_base[d] = (P)(ds + excl - dstart);
+        }
+        lds_barrier();
+
+        // keys out (digits remembered, four per register)
+        U* kout = job.kout;
+        V* vout = job.vout;
+        unsigned dg4[ITEMS / 4];
+    #pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const unsigned j = (unsigned)(r * BLOCK) + tid;
+            const U kb = s_stage[j];
+            const unsigned d = (sort_key_k<KIND, U>(kb, kind, flip) >> shift) & 255u;
+            if (r % 4 == 0) dg4[r / 4] = d;
+            else dg4[r / 4] |= d << (8 * (r % 4));
+            const P gpos = s_base[d] + j;
+            // gpos < n by construction; the check keeps a logic error from writing outside the output
+            if ((full || j < valid) && gpos < (P)n) kout[gpos] = kb;
+            if (r % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+        if (!VB) {
+            lds_barrier();  // the staging buffer is read until here
+            continue;
+        }
+        // 6. values staged in the same buffer and written out
+        lds_barrier();
+        unsigned spos0 = (unsigned)upos0 + l32;
+        __asm__ volatile("" : "+v"(spos0));
+    #pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            unsigned p = spos0 + (unsigned)(k * 32);
+            if (full || p < valid) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = val[k];
+        }
+        lds_barrier();
+        // slot indices and digit addresses recomputed here, not kept live from the key write-out
+        unsigned tid2 = tid;
+        __asm__ volatile("" : "+v"(tid2));
+    #pragma unroll
+        for (int i = 0; i < ITEMS / 4; i++) __asm__ volatile("" : "+v"(dg4[i]));
+    #pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const unsigned j = (unsigned)(r * BLOCK) + tid2;
+            const V vb = s_stage[j];
+            const unsigned d = (dg4[r / 4] >> (8 * (r % 4))) & 255u;
+            const P gpos = s_base[d] + j;
+            if ((full || j < valid) && gpos < (P)n) vout[gpos] = vb;
+        }
+
+        lds_barrier();  // the staging buffer is read until here
     }
 }
-
 
 // One long-tile digit pass over the whole input of a sort (buffers and
 // ranking mode from the plan; a pass over a constant digit returns at once).

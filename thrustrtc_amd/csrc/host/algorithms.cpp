@@ -156,13 +156,13 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     const unsigned kRadixBlock = n / (1024 * (size_t)ITEMS) >= cus ? 1024u : 512u;
     // 4-byte keys with 4-byte or no values: the long-tile pass
     // (trtc_radix_pass_long_k, TRTC_LONG_BLOCK x TRTC_LONG_ITEMS = 1024 x 32
-    // keys in the device header) for the largest sorts only; its grid is
-    // padded to whole runs of 8 tiles on each of the 8 XCDs
-    constexpr size_t kLongTile = 1024 * 32, kLongGridUnit = 64;
+    // keys in the device header) for the largest sorts only; one workgroup
+    // per CU loops over the tiles (long_pass_grid)
+    constexpr size_t kLongTile = 1024 * 32;
     const bool long_tiles = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n >= (3ull << 26) && n / kLongTile >= cus;
     const size_t TILE = long_tiles ? kLongTile : (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
-    const size_t grid = long_tiles ? (tiles + kLongGridUnit - 1) / kLongGridUnit * kLongGridUnit : tiles;
+    const size_t grid = long_tiles ? long_pass_grid(tiles) : tiles;
     const bool wide = n >= (1ull << 30);
     const size_t sw = wide ? 8 : 4;
 
@@ -795,6 +795,18 @@ bool inclusive_scan_fixed_scratch(const DVVectorLike& vin, DVVectorLike& vout, c
 {
     if (!scratch) return false;
     return general_scan(vin, vout, op, nullptr, false, scratch, scratch_bytes);
+}
+
+// Grid of the long-tile pass (trtc_radix_pass_long_k, trtc_seg_radix_pass_long_k):
+// its workgroups take tiles until none is left, tile runs going to virtual
+// XCD blockIdx.x mod 8, so the grid is a multiple of 8 (every virtual XCD has
+// workgroups) and at most one workgroup per CU (the pass holds 145 KB of LDS):
+// a workgroup moves to its next tile without a launch in between.
+size_t long_pass_grid(size_t tiles)
+{
+    const size_t cus = (size_t)std::max(8, context().num_cus);
+    const size_t want = std::min((tiles + 7) / 8, cus / 8);
+    return 8 * std::max<size_t>(1, want);
 }
 
 bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,

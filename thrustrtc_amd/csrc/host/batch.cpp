@@ -322,10 +322,10 @@ bool enqueue_sort(const SortPlan& plan, char* scratch, hipStream_t s)
         }
         KernelRef pk;
         if (!get_template_kernel(seg_pass_name(plan, C), "", pk)) return false;
-        // the long-tile pass hands tiles to the 8 XCDs in runs of 8 (a grid of
-        // whole runs, 8 tile counters per pass)
+        // the long-tile pass hands tiles to the 8 XCDs in runs of 8 (8 tile
+        // counters per pass) to workgroups that loop until none is left
         const bool lt = is_long_class(plan.KB, plan.VB, C.block, C.items);
-        const unsigned grid = lt ? (C.num_tiles + 63u) / 64u * 64u : C.num_tiles;
+        const unsigned grid = lt ? (unsigned)long_pass_grid(C.num_tiles) : C.num_tiles;
         for (int p = 0; p < plan.KB; p++) {
             a.pass = p;
             a.shift = 8 * p;

@@ -185,6 +185,9 @@ bool inclusive_scan_fixed_scratch(const DVVectorLike& vin, DVVectorLike& vout, c
 
 // Single-call onesweep radix sort (the TRTC_Sort_By_Key path) reading its
 // input from keys_src/vals_src and leaving the result in keys/vals.
+// Grid of the long-tile radix passes for `tiles` tiles (one workgroup per CU,
+// a multiple of 8).
+size_t long_pass_grid(size_t tiles);
 bool radix_sort_from(const void* keys_src, const void* vals_src, void* keys, void* vals, size_t n, int key_bytes,
                      int kind, bool descending, int val_bytes, hipStream_t s);
 
