@@ -697,14 +697,28 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
     __shared__ __attribute__((aligned(16))) unsigned s_stage[TILE];
     __shared__ unsigned short s_off[UNITS][256];  // unit's first local slot per digit
     __shared__ P s_base[256];
-    __shared__ unsigned s_dscan[4];
+    __shared__ unsigned s_dscan[BLOCK / 64];
+    __shared__ unsigned s_real[256];              // the tile's digit counts
+    __shared__ unsigned s_dstart[256];            // the tile's local digit offsets
     __shared__ unsigned s_tile;
     unsigned long long* const s_rank = (unsigned long long*)s_stage;
 
+    // the next tile of this workgroup: runs of XRUN consecutive tiles go to
+    // virtual XCD blockIdx.x mod 8 (thread 0; read by all after a barrier)
+    auto claim = [&]() {
+        const unsigned x = blockIdx.x & 7u;
+        const unsigned k = atomicAdd(tile_counter + x, 1u);
+        s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
+    };
     // Tiles until none is left: a grid of one workgroup per CU loops over them
     // (no workgroup launch between tiles), a grid of one per tile takes one
-    // each.  The thread index is made opaque per tile so that per-item
-    // addresses are recomputed instead of held across the loop.
+    // each.  Each tile claims its successor while it stages its values (keys
+    // only: once its lookback is done), so that the claim's round trip is off
+    // the critical path; a claimed tile only waits for its workgroup's current
+    // tile, which is lower, so every wait still points to a lower tile.  The
+    // thread index is made opaque per tile so that per-item addresses are
+    // recomputed instead of held across the loop.
+    if (threadIdx.x == 0) claim();
     for (;;) {
         unsigned tid = threadIdx.x;
         __asm__ volatile("" : "+v"(tid));
@@ -713,11 +727,6 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
             v4u* z = (v4u*)s_rank;
             const v4u zero = {0u, 0u, 0u, 0u};
             for (int i = tid; i < UNITS * 256 / 2; i += BLOCK) z[i] = zero;
-        }
-        if (tid == 0) {
-            const unsigned x = blockIdx.x & 7u;
-            const unsigned k = atomicAdd(tile_counter + x, 1u);
-            s_tile = (k / XRUN) * (8u * XRUN) + x * XRUN + k % XRUN;
         }
         lds_barrier();
         if ((size_t)s_tile >= num_tiles) break;  // no tiles left (or padding of the grid)
@@ -824,38 +833,62 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
         for (int k = 0; k < ITEMS; k++) __asm__ volatile("" : "+v"(key[k]));
         lds_barrier();
 
-        // 3. counts, first lookback reads, local digit scan
+        // 3. counts and local digit offsets, four threads per digit (adjacent
+        //    lanes, eight units each); the first lookback reads in flight (one
+        //    thread per digit)
+        static_assert(BLOCK == 1024 && UNITS == 32, "four threads per digit, eight units each");
         S w[LBB];
-        unsigned dstart = 0, real = 0;
         if (tid < 256) {
-            const unsigned d = tid;
-    #pragma unroll
+#pragma unroll
             for (int b = 0; b < LBB; b++) {
                 long long j = (long long)tile - 1 - b;
-                w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + d) : (S)0;
+                w[b] = (!head && j >= 0) ? ld_relaxed(st + (size_t)j * 256 + tid) : (S)0;
             }
-            unsigned run = 0;
-    #pragma unroll
-            for (int u = 0; u < UNITS; u++) {
-                const unsigned c = ((const unsigned*)(s_rank + u * 256 + d))[1];
-                s_off[u][d] = (unsigned short)run;
-                run += c;
+        }
+        {
+            const unsigned d = tid >> 2, q = tid & 3u;
+            unsigned pre[8];  // offsets inside this thread's eight units
+            unsigned part = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                pre[i] = part;
+                part += ((const unsigned*)(s_rank + (8 * q + i) * 256 + d))[1];
             }
-            real = (d == 255 && !full) ? run - (unsigned)(TILE - valid) : run;
-            if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+            // exclusive prefix over the digit's four threads, and the digit's
+            // count (the tile histogram) from its fourth thread
+            unsigned incl = part;
+            {
+                const unsigned y1 = (unsigned)__shfl_up((int)incl, 1, 4);
+                if (q >= 1) incl += y1;
+                const unsigned y2 = (unsigned)__shfl_up((int)incl, 2, 4);
+                if (q >= 2) incl += y2;
+            }
+            const unsigned total = (unsigned)__shfl((int)incl, (int)(lane | 3u), 64);
+            const unsigned excl_q = incl - part;
+            // published at once ("aggregate"; the first tile publishes its
+            // counts as inclusive prefixes)
+            const unsigned real = (d == 255 && !full) ? total - (unsigned)(TILE - valid) : total;
+            if (q == 0) {
+                if (st) st_relaxed(st + tile * 256 + d, (S)((head ? SB::incl : SB::agg) | (S)real));
+                s_real[d] = real;
+            }
+            // local digit scan: 16 digits per wave (their totals in the lanes
+            // with q == 0), then across the waves
             Plus plus;
-            unsigned inc = wave_inclusive_scan(run, plus);
-            if (lane == 63) s_dscan[wave] = inc;
-            dstart = inc - run;
+            const unsigned own = q == 0 ? total : 0u;
+            const unsigned wincl = wave_inclusive_scan(own, plus);
+            if (lane == 63) s_dscan[wave] = wincl;
+            lds_barrier();
+            // the digit's exclusive offset inside the wave, from its q == 0 lane
+            unsigned ds = (unsigned)__shfl((int)(wincl - own), (int)(lane & ~3u), 64);
+            for (unsigned ww = 0; ww < wave; ww++) ds += s_dscan[ww];
+            if (q == 0) s_dstart[d] = ds;
+#pragma unroll
+            for (int i = 0; i < 8; i++) s_off[8 * q + i][d] = (unsigned short)(pre[i] + excl_q + ds);
         }
         lds_barrier();
-        if (tid < 256) {
-            const unsigned d = tid;
-            for (unsigned ww = 0; ww < wave; ww++) dstart += s_dscan[ww];
-    #pragma unroll
-            for (int u = 0; u < UNITS; u++) s_off[u][d] = (unsigned short)(s_off[u][d] + dstart);
-        }
-        lds_barrier();
+        const unsigned dstart = tid < 256 ? s_dstart[tid] : 0u;
+        const unsigned real = tid < 256 ? s_real[tid] : 0u;
 
         // 4. local positions replace the ranks; keys into the staging buffer (the
         //    rank words are dead)
@@ -922,6 +955,7 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
                                         : (job.dstart32 ? (unsigned long long)job.dstart32[d] : (unsigned long long)dstart);
             s_base[d] = (P)(ds + excl - dstart);
         }
+        if (!VB && tid == 0) claim();  // keys only: the next tile, while this one writes out
         lds_barrier();
 
         // keys out (digits remembered, four per register)
@@ -953,6 +987,7 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
             unsigned p = spos0 + (unsigned)(k * 32);
             if (full || p < valid) s_stage[(lp2[k / 2] >> (16 * (k & 1))) & 0xFFFFu] = val[k];
         }
+        if (tid == 0) claim();  // the next tile, while this one writes its values out
         lds_barrier();
         // slot indices and digit addresses recomputed here, not kept live from the key write-out
         unsigned tid2 = tid;
