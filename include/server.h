@@ -38,12 +38,19 @@ class THRUST_RTC_API TRTC_Server
 {
 public:
     // hbm_budget: bytes of outputs and temporaries the server may hold at
-    // once; slots: batches in flight at once (streams); batching false: one
-    // request at a time through the single-call path.
+    // once, reserved in device memory when the server starts (its buffer
+    // pool; grown on demand if that reservation fails); slots: batches in
+    // flight at once (streams); batching false: one request at a time through
+    // the single-call path.
     TRTC_Server(size_t hbm_budget, int slots, bool batching);
     ~TRTC_Server();
     TRTC_Server(const TRTC_Server&) = delete;
     TRTC_Server& operator=(const TRTC_Server&) = delete;
+
+    // Batch size limits of the batching server: a batch closes at
+    // max_requests requests or before its elements pass max_elements (it
+    // always takes at least one request).  Defaults 4096 and 2^28.
+    void Set_Batch_Limits(size_t max_requests, size_t max_elements);
 
     // Queue a request; returns a ticket (>= 0) or -1 for an invalid request.
     // key_elem / val_elem are element type names ("uint32_t", "float", ...;

@@ -238,6 +238,8 @@ TRTC_CAPI unsigned long long n_batch_describe(void* batch, char* json_out, unsig
  * 2 Inclusive_Scan, 3 Reduce.  Submit returns a ticket or -1. */
 TRTC_CAPI void* n_server_create(unsigned long long hbm_budget, int slots, int batching);
 TRTC_CAPI void n_server_destroy(void* server);
+// Batch limits of a batching server (TRTC_Server::Set_Batch_Limits); no counterpart in the reference.
+TRTC_CAPI void n_server_set_batch_limits(void* server, unsigned long long max_requests, unsigned long long max_elements);
 TRTC_CAPI long long n_server_submit(void* server, int op, const char* key_elem, const char* val_elem, void* in_a,
                                     void* in_b, unsigned long long n);
 TRTC_CAPI int n_server_wait(void* server, long long ticket);

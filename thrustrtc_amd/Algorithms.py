@@ -466,6 +466,11 @@ class Server:
             native.n_server_destroy(self.m_cptr)
             self.m_cptr = None
 
+    def set_batch_limits(self, max_requests=4096, max_elements=1 << 28):
+        """A batch closes at max_requests requests or before its elements pass
+        max_elements (it always takes at least one request)."""
+        native.n_server_set_batch_limits(self.m_cptr, max_requests, max_elements)
+
     def submit(self, op, key_elem, in_a, n, val_elem=None, in_b=None):
         """in_a / in_b: device addresses (ints) or DVVector-likes."""
         def addr(x):

@@ -17,13 +17,25 @@ size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 const size_t kNoLimit = ~(size_t)0;
 }  // namespace
 
+// Host table -> device, asynchronously on stream s: staged in pinned memory
+// and copied by the 16-byte copy kernel reading the pinned buffer directly.
+// (hipMemcpyAsync from the same pinned buffer returned only once the copy
+// ran for tables of a few hundred KB, ~6 ms while another stream's sort held
+// every CU, with the GPU idle behind this batch meanwhile.)
 bool upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
 {
     if (!bytes) return true;
     void* st = pinned_alloc(bytes);
     if (!st) return false;
     memcpy(st, src, bytes);
-    bool ok = check_hip(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    void* dev_src = nullptr;
+    bool ok;
+    if (hipHostGetDevicePointer(&dev_src, st, 0) == hipSuccess && dev_src)
+        ok = copy_bytes(dev_src, dst, bytes, s);
+    else {
+        (void)hipGetLastError();
+        ok = check_hip(hipMemcpyAsync(dst, st, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    }
     pinned_release(st, s);
     return ok;
 }

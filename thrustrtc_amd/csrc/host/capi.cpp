@@ -850,6 +850,10 @@ void* n_server_create(unsigned long long hbm_budget, int slots, int batching)
     return new TRTC_Server((size_t)hbm_budget, slots, batching != 0);
 }
 void n_server_destroy(void* server) { delete (TRTC_Server*)server; }
+void n_server_set_batch_limits(void* server, unsigned long long max_requests, unsigned long long max_elements)
+{
+    ((TRTC_Server*)server)->Set_Batch_Limits((size_t)max_requests, (size_t)max_elements);
+}
 long long n_server_submit(void* server, int op, const char* key_elem, const char* val_elem, void* in_a, void* in_b,
                           unsigned long long n)
 {

@@ -167,6 +167,7 @@ SIGNATURES = {
     "n_batch_describe": (_ull, [_vp, ctypes.c_char_p, _ull]),
     "n_server_create": (_vp, [_ull, _i, _i]),
     "n_server_destroy": (None, [_vp]),
+    "n_server_set_batch_limits": (None, [_vp, _ull, _ull]),
     "n_server_submit": (ctypes.c_longlong, [_vp, _i, _s, _s, _vp, _vp, _ull]),
     "n_server_wait": (_i, [_vp, ctypes.c_longlong]),
     "n_server_done": (_i, [_vp, ctypes.c_longlong]),
