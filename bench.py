@@ -685,7 +685,7 @@ def main():
     log(f"Sort_By_Key 2^{a.n} u32 uniform: {ms:.3f} ms/step, {gkeys:.2f} Gkeys/s, kernels: {prof}")
 
     # dominant kernel: one onesweep digit pass (reads 8 B/pair, writes 8 B/pair);
-    # from 3*2^26 keys the long-tile pass, below it the 1024x16 pass
+    # from 2^23 keys (one tile per CU) the long-tile pass, below it the 1024x16 pass
     pass_name = max((k for k in prof if k.startswith(("trtc_radix_pass_long_k<4,", "trtc_radix_pass_k<4, 4,"))),
                     key=lambda k: prof[k]["total_ms"])
     pass_ms = prof[pass_name]["total_ms"] / prof[pass_name]["count"]

@@ -233,8 +233,9 @@ def test_batched_segments_4096_workload(trtc, wl, schedule):
 
 
 # ---------------------------------------------------------------- long-tile pass
-# Sorts of at least 3 * 2^26 4-byte keys with 4-byte or no values take the
-# 1024 x 24-key tile pass.  Its key kinds, descending order, keys-only form,
+# Sorts of 4-byte keys with 4-byte or no values take the 1024 x 32-key tile
+# pass once every CU gets a tile (2^23 keys on 256 CUs); 3 * 2^26 + 12345 keys
+# run it with many tiles per workgroup.  Its key kinds, descending order, keys-only form,
 # partial last tile and skipped constant digits are checked here by the
 # properties that define a stable sort, in O(n) host work: the output keys are
 # ordered, the payload is a permutation of the indices, every output key is

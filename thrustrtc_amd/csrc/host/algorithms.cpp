@@ -156,10 +156,12 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     const unsigned kRadixBlock = n / (1024 * (size_t)ITEMS) >= cus ? 1024u : 512u;
     // 4-byte keys with 4-byte or no values: the long-tile pass
     // (trtc_radix_pass_long_k, TRTC_LONG_BLOCK x TRTC_LONG_ITEMS = 1024 x 32
-    // keys in the device header) for the largest sorts only; one workgroup
-    // per CU loops over the tiles (long_pass_grid)
+    // keys in the device header) once every CU gets a tile; one workgroup per
+    // CU loops over the tiles (long_pass_grid).  u32/u32 sorts, long tiles
+    // against 1024x16 (one box): 2^22 0.165 vs 0.134 ms, 2^23 0.187 vs 0.211,
+    // 2^24 0.325 vs 0.369, 2^26 1.13 vs 1.28, 2^27 2.10 vs 2.46
     constexpr size_t kLongTile = 1024 * 32;
-    const bool long_tiles = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n >= (3ull << 26) && n / kLongTile >= cus;
+    const bool long_tiles = KB == 4 && (val_bytes == 0 || val_bytes == 4) && n / kLongTile >= cus;
     const size_t TILE = long_tiles ? kLongTile : (size_t)kRadixBlock * ITEMS;
     const size_t tiles = (n + TILE - 1) / TILE;
     const size_t grid = long_tiles ? long_pass_grid(tiles) : tiles;
