@@ -147,7 +147,7 @@ template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigne
 // keys before the first line boundary and after the last 16-byte piece are
 // counted one per thread (nthreads >= 128 / sizeof(key)).
 template <int KB, int UNR>
-TRTC_DEV void hist_count_keys(const typename KeyWord<KB>::U* keys, size_t n, size_t tid, size_t nthreads,
+TRTC_DEV void hist_count_keys(const gmem_t<typename KeyWord<KB>::U>* keys, size_t n, size_t tid, size_t nthreads,
                               unsigned* hcol, int kind, typename KeyWord<KB>::U flip)
 {
     typedef typename KeyWord<KB>::U U;
@@ -173,7 +173,7 @@ TRTC_DEV void hist_count_keys(const typename KeyWord<KB>::U* keys, size_t n, siz
     size_t head = mis ? LINE_KEYS - mis : 0;
     if (head > n) head = n;
     if (tid < head) count1(keys[tid]);
-    const v4u* kv = (const v4u*)(keys + head);
+    const gmem_t<const v4u>* kv = (const gmem_t<const v4u>*)(keys + head);
     const size_t nvec = (n - head) / PER_VEC;
     const v4u zero = {0u, 0u, 0u, 0u};
     v4u cur[UNR];
@@ -230,7 +230,7 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     const U flip = a.descending ? (U)~(U)0 : (U)0;
     const size_t stride = (size_t)gridDim.x * BLOCK;
     const size_t gid = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    hist_count_keys<KB, 4>((const U*)a.keys, a.n, gid, stride, hcol, a.kind, flip);
+    hist_count_keys<KB, 4>(gmem((const U*)a.keys), a.n, gid, stride, hcol, a.kind, flip);
     {
         // clear the passes' lookback state, 16 bytes per store when aligned
         size_t z0 = 0;
@@ -392,8 +392,8 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     const unsigned valid = (unsigned)(tile_end - tile_base);
     const bool full = valid == (unsigned)TILE;
     const U flip = descending ? (U)~(U)0 : (U)0;
-    const U* kin = job.kin + tile_base;
-    const V* vin = job.vin + tile_base;
+    const gmem_t<const U>* kin = gmem(job.kin + tile_base);
+    const gmem_t<const V>* vin = gmem(job.vin + tile_base);
     const int wpos0 = (int)wave * 64 * ITEMS;
 
     // 1. loads
@@ -556,7 +556,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     // 6. write each digit run to its global slot.  All LDS reads are issued
     //    before the first store (separate loops), so they overlap instead of
     //    waiting one key at a time.
-    U* kout = job.kout;
+    gmem_t<U>* kout = gmem(job.kout);
     U kb[ITEMS];
     P gpos[ITEMS];
 #pragma unroll
@@ -574,7 +574,7 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
         if ((full || j < valid) && gpos[r] < (P)n) kout[gpos[r]] = kb[r];
     }
     if (VB) {
-        V* vout = job.vout;
+        gmem_t<V>* vout = gmem(job.vout);
         V vb[ITEMS];
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) vb[r] = s_vals[(unsigned)(r * BLOCK) + threadIdx.x];
@@ -744,8 +744,8 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
         const unsigned lane = lane_id();
         const unsigned l32 = lane & 31u;
         const unsigned unit = tid >> 5;
-        const U* kin = job.kin + tile_base;
-        const V* vin = job.vin + tile_base;
+        const gmem_t<const U>* kin = gmem(job.kin + tile_base);
+        const gmem_t<const V>* vin = gmem(job.vin + tile_base);
         const int upos0 = (int)unit * UITEMS;
         const U flip = descending ? ~0u : 0u;
         S* st = job.status;
@@ -959,20 +959,34 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
         lds_barrier();
 
         // keys out (digits remembered, four per register)
-        U* kout = job.kout;
-        V* vout = job.vout;
+        gmem_t<U>* kout = gmem(job.kout);
+        gmem_t<V>* vout = gmem(job.vout);
         unsigned dg4[ITEMS / 4];
+        // groups of WB items: their LDS reads issue together, then their stores
+        // (one item at a time, each store's branch kept the next item's LDS
+        // reads behind it)
+        constexpr int WB = 8;
     #pragma unroll
-        for (int r = 0; r < ITEMS; r++) {
-            const unsigned j = (unsigned)(r * BLOCK) + tid;
-            const U kb = s_stage[j];
-            const unsigned d = (sort_key_k<KIND, U>(kb, kind, flip) >> shift) & 255u;
-            if (r % 4 == 0) dg4[r / 4] = d;
-            else dg4[r / 4] |= d << (8 * (r % 4));
-            const P gpos = s_base[d] + j;
-            // gpos < n by construction; the check keeps a logic error from writing outside the output
-            if ((full || j < valid) && gpos < (P)n) kout[gpos] = kb;
-            if (r % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+        for (int r0 = 0; r0 < ITEMS; r0 += WB) {
+            U kb[WB];
+            P gp[WB];
+    #pragma unroll
+            for (int i = 0; i < WB; i++) kb[i] = s_stage[(unsigned)((r0 + i) * BLOCK) + tid];
+    #pragma unroll
+            for (int i = 0; i < WB; i++) {
+                const int r = r0 + i;
+                const unsigned d = (sort_key_k<KIND, U>(kb[i], kind, flip) >> shift) & 255u;
+                if (r % 4 == 0) dg4[r / 4] = d;
+                else dg4[r / 4] |= d << (8 * (r % 4));
+                gp[i] = s_base[d] + (P)((unsigned)(r * BLOCK) + tid);
+            }
+    #pragma unroll
+            for (int i = 0; i < WB; i++) {
+                const unsigned j = (unsigned)((r0 + i) * BLOCK) + tid;
+                // gpos < n by construction; the check keeps a logic error from writing outside the output
+                if ((full || j < valid) && gp[i] < (P)n) kout[gp[i]] = kb[i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (!VB) {
             lds_barrier();  // the staging buffer is read until here
@@ -995,12 +1009,23 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
     #pragma unroll
         for (int i = 0; i < ITEMS / 4; i++) __asm__ volatile("" : "+v"(dg4[i]));
     #pragma unroll
-        for (int r = 0; r < ITEMS; r++) {
-            const unsigned j = (unsigned)(r * BLOCK) + tid2;
-            const V vb = s_stage[j];
-            const unsigned d = (dg4[r / 4] >> (8 * (r % 4))) & 255u;
-            const P gpos = s_base[d] + j;
-            if ((full || j < valid) && gpos < (P)n) vout[gpos] = vb;
+        for (int r0 = 0; r0 < ITEMS; r0 += WB) {
+            V vb[WB];
+            P gp[WB];
+    #pragma unroll
+            for (int i = 0; i < WB; i++) vb[i] = s_stage[(unsigned)((r0 + i) * BLOCK) + tid2];
+    #pragma unroll
+            for (int i = 0; i < WB; i++) {
+                const int r = r0 + i;
+                const unsigned d = (dg4[r / 4] >> (8 * (r % 4))) & 255u;
+                gp[i] = s_base[d] + (P)((unsigned)(r * BLOCK) + tid2);
+            }
+    #pragma unroll
+            for (int i = 0; i < WB; i++) {
+                const unsigned j = (unsigned)((r0 + i) * BLOCK) + tid2;
+                if ((full || j < valid) && gp[i] < (P)n) vout[gp[i]] = vb[i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
 
         lds_barrier();  // the staging buffer is read until here
@@ -1057,16 +1082,16 @@ TRTC_DEV void radix_copy_span(const void* src, void* dst, size_t bytes)
     const size_t al = (size_t)src | (size_t)dst | bytes;
     if ((al & 15) == 0) {
         typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        const v4u* s = (const v4u*)src;
-        v4u* d = (v4u*)dst;
+        const gmem_t<const v4u>* s = gmem((const v4u*)src);
+        gmem_t<v4u>* d = gmem((v4u*)dst);
         for (size_t i = gid; i < bytes / 16; i += stride) d[i] = s[i];
     } else if ((al & 3) == 0) {
-        const unsigned* s = (const unsigned*)src;
-        unsigned* d = (unsigned*)dst;
+        const gmem_t<const unsigned>* s = gmem((const unsigned*)src);
+        gmem_t<unsigned>* d = gmem((unsigned*)dst);
         for (size_t i = gid; i < bytes / 4; i += stride) d[i] = s[i];
     } else {
-        const unsigned char* s = (const unsigned char*)src;
-        unsigned char* d = (unsigned char*)dst;
+        const gmem_t<const unsigned char>* s = gmem((const unsigned char*)src);
+        gmem_t<unsigned char>* d = gmem((unsigned char*)dst);
         for (size_t i = gid; i < bytes; i += stride) d[i] = s[i];
     }
 }
@@ -1178,7 +1203,7 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     const size_t base = (size_t)a.hist_chunk[2 * c + 1] * ck;
     const size_t cnt = (d.n - base) < ck ? (size_t)(d.n - base) : ck;
     const U flip = a.descending ? (U)~(U)0 : (U)0;
-    hist_count_keys<KB, 4>((const U*)d.src_keys + base, cnt, threadIdx.x, BLOCK, hcol, a.kind, flip);
+    hist_count_keys<KB, 4>(gmem((const U*)d.src_keys + base), cnt, threadIdx.x, BLOCK, hcol, a.kind, flip);
     __syncthreads();
     unsigned* hist = a.seg_hist + (size_t)d.multi * KB * 256;
     for (int i = threadIdx.x; i < BINS; i += BLOCK) {
