@@ -309,6 +309,13 @@ TRTC_DEV void lds_barrier()
 
 template <class T> TRTC_DEV T stream_load(const T* p) { return __builtin_nontemporal_load(p); }
 
+// Buffer resource for `count` 4-byte words from p: raw buffer loads through it
+// return 0 at and past the end (no clamped addresses for a partial tile).
+TRTC_DEV __amdgpu_buffer_rsrc_t tile_rsrc(const void* p, unsigned count)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(count * 4u), 0x00020000);
+}
+
 // One digit pass.  Tile = BLOCK*ITEMS keys; item k of a lane sits at tile
 // position wave*64*ITEMS + k*64 + lane (coalesced striped loads), and ranks
 // are assigned in that position order, which keeps the pass stable.
@@ -752,18 +759,27 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
         const bool head = tile == 0;
         const bool lds_rank = job.lds_rank;
         // 1. keys (registers)
+        // Full tiles: one address per lane and constant offsets (k * 128 bytes)
+        // in the load instructions.  The partial last tile: buffer loads, which
+        // return 0 past its end, so that both paths issue the same 32 loads and
+        // the ranking can wait for them one at a time (clamped per-item indices
+        // made the compiler merge both paths into one set of loads with 32
+        // separate 64-bit addresses; loads under a branch made it wait for all
+        // 32 before the first rank).
         U key[ITEMS];
         unsigned in_range = ~0u;  // bit k: item k lies inside the input
         if (full) {
+            const gmem_t<const U>* const kl = kin + (upos0 + (int)l32);
     #pragma unroll
-            for (int k = 0; k < ITEMS; k++) key[k] = stream_load(kin + (upos0 + k * 32) + l32);
+            for (int k = 0; k < ITEMS; k++) key[k] = kl[k * 32];
         } else {
+            const __amdgpu_buffer_rsrc_t rs = tile_rsrc(job.kin + tile_base, valid);
+            const unsigned off = ((unsigned)upos0 + l32) * 4u;
             in_range = 0;
     #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
-                unsigned p = (unsigned)(upos0 + k * 32) + l32;
-                key[k] = kin[p < valid ? p : valid - 1];  // a clamped index: the loads issue together
-                in_range |= (p < valid ? 1u : 0u) << k;
+                key[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(off + k * 128), 0, 0);
+                in_range |= ((unsigned)(upos0 + k * 32) + l32 < valid ? 1u : 0u) << k;
             }
         }
         // digit of item k, recomputed from the key wherever it is needed
@@ -909,14 +925,13 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
             unsigned vpos0 = (unsigned)upos0 + l32;
             __asm__ volatile("" : "+v"(vpos0));
             if (full) {
+                const gmem_t<const V>* const vl = vin + vpos0;
     #pragma unroll
-                for (int k = 0; k < ITEMS; k++) val[k] = stream_load(vin + (vpos0 + k * 32));
+                for (int k = 0; k < ITEMS; k++) val[k] = vl[k * 32];
             } else {
+                const __amdgpu_buffer_rsrc_t rs = tile_rsrc(job.vin + tile_base, valid);
     #pragma unroll
-                for (int k = 0; k < ITEMS; k++) {
-                    unsigned p = vpos0 + (unsigned)(k * 32);
-                    val[k] = vin[p < valid ? p : valid - 1];
-                }
+                for (int k = 0; k < ITEMS; k++) val[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(vpos0 * 4u + k * 128), 0, 0);
             }
         }
         // 5. finish the lookback for digit d
