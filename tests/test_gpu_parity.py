@@ -263,6 +263,42 @@ def test_sort_keys_only_constant_digits(trtc, orc, wl, case):
     assert np.array_equal(dk.to_host(), orc.sort(keys, elem))
 
 
+@pytest.mark.parametrize("case", ["one_digit", "three_digits", "all_equal", "below_2^16"])
+def test_long_pass_constant_digits(trtc, orc, wl, case):
+    """In-place sorts on the long-tile pass (2^23 + 5 keys) whose constant
+    digits leave an odd number of sorting passes (one_digit, three_digits): the
+    parity copy runs in the launch of a constant digit's pass."""
+    n = (1 << 23) + 5
+    say(f"long-tile pass, constant digits {case} n={n}")
+    keys, elem = constant_digit_keys(wl, case, n, 19)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv)
+    perm = orc.sort_perm(keys, elem, False)
+    assert np.array_equal(dk.to_host(), keys[perm])
+    assert np.array_equal(dv.to_host(), perm.astype(np.uint32))
+    dk2 = trtc.device_vector_from_numpy(keys)
+    trtc.Sort(dk2, trtc.Greater())
+    assert np.array_equal(dk2.to_host(), orc.sort(keys, elem, True))
+
+
+@pytest.mark.parametrize("low_const", [True, False])
+@pytest.mark.parametrize("n", [100003, (1 << 22) + 9])
+def test_sort_u16_one_constant_digit(trtc, orc, wl, low_const, n):
+    """2-byte keys with one constant digit: one sorting pass in place, the
+    parity copy in the other digit's launch (before or after the pass)."""
+    say(f"u16 keys, one constant digit (low={low_const}) n={n}")
+    x = wl.u32_uniform(n, 23)
+    keys = ((x & np.uint32(0xFF00)) | np.uint32(0x5A)) if low_const else ((x & np.uint32(0xFF)) | np.uint32(0xA500))
+    keys = keys.astype(np.uint16)
+    dk = trtc.device_vector_from_numpy(keys)
+    dv = trtc.device_vector_from_numpy(np.arange(n, dtype=np.uint32))
+    trtc.Sort_By_Key(dk, dv)
+    perm = orc.sort_perm(keys, "uint16_t", False)
+    assert np.array_equal(dk.to_host(), keys[perm])
+    assert np.array_equal(dv.to_host(), perm.astype(np.uint32))
+
+
 # ---------------------------------------------------------------- scan
 SCAN_SIZES = [1, 2, 1000, 4095, 4096, 4097, 100003, 1 << 20]
 

@@ -157,6 +157,44 @@ def test_server_per_call_mode_matches_batched(trtc, arenas):
             assert all(u.tobytes() == v.tobytes() for u, v in zip(x, y)), r
 
 
+@pytest.mark.parametrize("n", [100003, 1 << 20, (1 << 23) + 5])
+def test_server_per_call_constant_digits(trtc, orc, wl, n):
+    """Per-call sorts write a separate output (out of place): with every digit
+    constant the source is copied in pass 0's launch; with one or three
+    sorting passes the first pass reads the source and the last writes the
+    output."""
+    say(f"server per-call: constant-digit keys, n = {n}")
+    x = wl.u32_uniform(n, 29)
+    cases = {"all_equal": np.full(n, 0xDEADBEEF, dtype=np.uint32),
+             "one_digit": (x & np.uint32(0xFF00)) | np.uint32(0x13000000),
+             "three_digits": (x & np.uint32(0xFFFF00FF)) | np.uint32(0x4200)}
+    srv = trtc.Server(64 << 30, 1, False)
+    vals = np.arange(n, dtype=np.uint32)
+    dvals = trtc.device_vector_from_numpy(vals)
+    for name, keys in cases.items():
+        dk = trtc.device_vector_from_numpy(keys)
+        pk = trtc.native.n_dvvectorlike_data(dk.m_cptr)
+        pv = trtc.native.n_dvvectorlike_data(dvals.m_cptr)
+        t = srv.submit(trtc.REQ_SORT_BY_KEY, "uint32_t", pk, n, "uint32_t", pv)
+        t2 = srv.submit(trtc.REQ_SORT, "uint32_t", pk, n)
+        srv.wait(t)
+        srv.wait(t2)
+        got = []
+        for tt, which in ((t, 0), (t, 1), (t2, 0)):
+            a = trtc.DVVectorAdaptor("uint32_t", n, srv.output(tt, which))
+            dst = trtc.device_vector("uint32_t", n)
+            trtc.Copy(a, dst)
+            got.append(dst.to_host())
+        srv.release(t)
+        srv.release(t2)
+        perm = orc.sort_perm(keys, "uint32_t")
+        assert np.array_equal(got[0], keys[perm]), name
+        assert np.array_equal(got[1], perm.astype(np.uint32)), name
+        assert np.array_equal(got[2], keys[perm]), name
+        # the input is left as it was
+        assert np.array_equal(dk.to_host(), keys), name
+
+
 def test_server_small_budget_admission(trtc, orc, arenas):
     say("server: 32 MiB budget, 80 requests")
     rng = np.random.default_rng(4)

@@ -61,6 +61,7 @@ struct RadixPlan
     unsigned long long kin[8], kout[8], vin[8], vout[8];
     unsigned long long copy_ksrc, copy_kdst, copy_vsrc, copy_vdst;
     unsigned rank_lds;
+    unsigned copy_mask;  // bit p: pass p's launch copies kin -> kout (its digit is constant)
 };
 
 struct RadixHistArgs

@@ -98,12 +98,15 @@ TRTC_DEV void st_relaxed(unsigned long long* p, unsigned long long v)
 }
 
 // The passes that run read and write alternately the ping-pong and the
-// caller's buffers so that, where possible, the last one writes the caller's
-// buffers: out of place (source != result) the first pass reads the source
-// and the parity is chosen backwards from the last pass; in place, the first
-// pass must leave the source, so an odd number of passes ends in the
-// ping-pong buffers and a copy brings the result back.  No pass runs when
-// every digit is constant (then only an out-of-place sort copies).
+// caller's buffers so that the last one writes the caller's buffers: out of
+// place (source != result) the first pass reads the source and the parity is
+// chosen backwards from the last pass; in place, the first pass must leave
+// the source, so an odd number of sorting passes takes one copy, done by the
+// launch of a pass whose digit is constant (bit p of copy_mask: pass p copies
+// kin -> kout instead of sorting), which exists whenever KB is even.  With
+// every digit constant an out-of-place sort copies the source in pass 0's
+// launch.  Only a 1-byte-key sort in place has no spare slot: its copy is
+// left to radix_copyback (copy_ksrc / copy_kdst).
 template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigned constant_mask, unsigned rank_lds)
 {
     RadixPlan& P = *a.plan;
@@ -114,19 +117,29 @@ template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigne
     const bool in_place = ks == kd;
     int runs = 0;
     for (int p = 0; p < KB; p++) runs += (constant_mask >> p) & 1u ? 0 : 1;
+    // the slot of the parity copy: the first constant digit (in place with an
+    // odd number of sorting passes), pass 0 (out of place, every digit constant)
+    int copy_slot = -1;
+    if (in_place ? (runs & 1) != 0 : runs == 0) {
+        for (int p = 0; p < KB && copy_slot < 0; p++)
+            if ((constant_mask >> p) & 1u) copy_slot = p;
+    }
+    const int ops = runs + (copy_slot >= 0 ? 1 : 0);
     unsigned long long ck = ks, cv = vs;  // current location of the data
     int i = 0;
+    P.copy_mask = 0;
     for (int p = 0; p < 8; p++) {
         P.kin[p] = P.kout[p] = P.vin[p] = P.vout[p] = 0;
-        if (p >= KB || ((constant_mask >> p) & 1u)) continue;
-        // pass number i of `runs`: write the result buffers when the passes
-        // left after this one are even in number (out of place), or
+        if (p >= KB || (((constant_mask >> p) & 1u) && p != copy_slot)) continue;
+        // operation number i of `ops`: write the result buffers when the
+        // operations left after this one are even in number (out of place), or
         // alternate starting with the ping-pong buffers (in place)
-        const bool to_dst = in_place ? (i & 1) != 0 : ((runs - 1 - i) & 1) == 0;
+        const bool to_dst = in_place ? (i & 1) != 0 : ((ops - 1 - i) & 1) == 0;
         P.kin[p] = ck;
         P.vin[p] = cv;
         P.kout[p] = to_dst ? kd : kt;
         P.vout[p] = to_dst ? vd : vt;
+        if (p == copy_slot) P.copy_mask |= 1u << p;
         ck = P.kout[p];
         cv = P.vout[p];
         i++;
@@ -314,6 +327,30 @@ template <class T> TRTC_DEV T stream_load(const T* p) { return __builtin_nontemp
 TRTC_DEV __amdgpu_buffer_rsrc_t tile_rsrc(const void* p, unsigned count)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(count * 4u), 0x00020000);
+}
+
+// A grid-stride copy of `bytes` bytes, 16 per step when both buffers and the
+// length allow, else 4 or 1: the copy a sort's plan asks for (in a pass slot
+// whose digit is constant, or after the passes of a 1-byte-key sort).
+TRTC_DEV void radix_copy_span(const void* src, void* dst, size_t bytes)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t al = (size_t)src | (size_t)dst | bytes;
+    if ((al & 15) == 0) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const gmem_t<const v4u>* s = gmem((const v4u*)src);
+        gmem_t<v4u>* d = gmem((v4u*)dst);
+        for (size_t i = gid; i < bytes / 16; i += stride) d[i] = s[i];
+    } else if ((al & 3) == 0) {
+        const gmem_t<const unsigned>* s = gmem((const unsigned*)src);
+        gmem_t<unsigned>* d = gmem((unsigned*)dst);
+        for (size_t i = gid; i < bytes / 4; i += stride) d[i] = s[i];
+    } else {
+        const gmem_t<const unsigned char>* s = gmem((const unsigned char*)src);
+        gmem_t<unsigned char>* d = gmem((unsigned char*)dst);
+        for (size_t i = gid; i < bytes; i += stride) d[i] = s[i];
+    }
 }
 
 // One digit pass.  Tile = BLOCK*ITEMS keys; item k of a lane sits at tile
@@ -612,6 +649,11 @@ TRTC_DEV void radix_pass(const RadixPassArgs& a)
         vin = (const void*)P.vin[a.pass];
         kout = (void*)P.kout[a.pass];
         vout = (void*)P.vout[a.pass];
+        if ((P.copy_mask >> a.pass) & 1u) {
+            radix_copy_span(kin, kout, a.n * (size_t)KB);
+            if (VB) radix_copy_span(vin, vout, a.n * (size_t)VB);
+            return;
+        }
     }
     auto make = [&](unsigned t) {
         RadixJob<U, V, S> j;
@@ -1064,6 +1106,11 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
         vin = (const void*)PL.vin[a.pass];
         kout = (void*)PL.kout[a.pass];
         vout = (void*)PL.vout[a.pass];
+        if ((PL.copy_mask >> a.pass) & 1u) {
+            radix_copy_span(kin, kout, a.n * 4);
+            if (VB) radix_copy_span(vin, vout, a.n * (size_t)VB);
+            return;
+        }
         lds_rank = (PL.rank_lds >> a.pass) & 1u;
     }
     constexpr size_t TILE = (size_t)BLOCK * ITEMS;
@@ -1084,31 +1131,6 @@ TRTC_DEV void radix_pass_long(const RadixPassArgs& a)
     };
     radix_pass_long_core<BLOCK, ITEMS, VB, S, KIND>(a.tile_counter, (a.n + TILE - 1) / TILE, make, a.shift, a.kind,
                                                      a.descending);
-}
-
-// The copy a plan may ask for after the passes (the result of an odd number
-// of in-place passes lies in the ping-pong buffers): grid-stride, 16 bytes
-// per step when both buffers and the length allow, else 4 or 1.  Returns at
-// once when the plan needs no copy.
-TRTC_DEV void radix_copy_span(const void* src, void* dst, size_t bytes)
-{
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t al = (size_t)src | (size_t)dst | bytes;
-    if ((al & 15) == 0) {
-        typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        const gmem_t<const v4u>* s = gmem((const v4u*)src);
-        gmem_t<v4u>* d = gmem((v4u*)dst);
-        for (size_t i = gid; i < bytes / 16; i += stride) d[i] = s[i];
-    } else if ((al & 3) == 0) {
-        const gmem_t<const unsigned>* s = gmem((const unsigned*)src);
-        gmem_t<unsigned>* d = gmem((unsigned*)dst);
-        for (size_t i = gid; i < bytes / 4; i += stride) d[i] = s[i];
-    } else {
-        const gmem_t<const unsigned char>* s = gmem((const unsigned char*)src);
-        gmem_t<unsigned char>* d = gmem((unsigned char*)dst);
-        for (size_t i = gid; i < bytes; i += stride) d[i] = s[i];
-    }
 }
 
 TRTC_DEV void radix_copyback(const RadixCopyArgs& a)
