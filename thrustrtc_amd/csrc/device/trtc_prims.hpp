@@ -226,12 +226,13 @@ TRTC_DEV T lookback(const LookbackArgs& lb, size_t tile, Op& op, size_t first = 
     return acc;
 }
 
-// Raw-pointer view of one segment of a batch.
+// Raw-pointer view of one segment of a batch (device memory whose address
+// comes from the batch's descriptor table: accessed as global memory).
 template <class T> struct PtrView
 {
     typedef T value_t;
     T* p;
-    TRTC_DEV T& operator[](size_t i) const { return p[i]; }
+    TRTC_DEV gmem_t<T>& operator[](size_t i) const { return gmem(p)[i]; }
 };
 
 // Raw element pointer of contiguous views (VectorView, PtrView), for the
@@ -302,7 +303,7 @@ template <int ITEMS, class T> TRTC_DEV void load_striped_vec(const T* p, T* lds,
     const unsigned lane = lane_id();
     trtc_v4u v[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load((const trtc_v4u*)p + k * 64 + lane);
+    for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(gmem((const trtc_v4u*)p) + k * 64 + lane);
     wave_sync();  // the buffer's previous contents are consumed
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -332,7 +333,7 @@ template <int ITEMS, class T> TRTC_DEV void load_striped_vec_shifted(const T* p,
     static_assert(NV * PER == ITEMS, "whole 16-byte pieces per lane");
     const unsigned lane = lane_id();
     const int m = (int)(((size_t)p & 15) / sizeof(T));  // 1 .. PER-1 elements past the boundary
-    const trtc_v4u* q = (const trtc_v4u*)((const char*)p - (size_t)m * sizeof(T));
+    const gmem_t<const trtc_v4u>* q = gmem((const trtc_v4u*)((const char*)p - (size_t)m * sizeof(T)));
     trtc_v4u v[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = __builtin_nontemporal_load(q + k * 64 + lane);

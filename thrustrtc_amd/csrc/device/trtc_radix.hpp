@@ -159,14 +159,13 @@ template <int KB> TRTC_DEV void write_radix_plan(const RadixHistArgs& a, unsigne
 // group is counted (one group at a time leaves the read latency-bound); the
 // keys before the first line boundary and after the last 16-byte piece are
 // counted one per thread (nthreads >= 128 / sizeof(key)).
-template <int KB, int UNR>
+template <int KB, int UNR, int C = (KB <= 4 ? 32 : 16)>
 TRTC_DEV void hist_count_keys(const gmem_t<typename KeyWord<KB>::U>* keys, size_t n, size_t tid, size_t nthreads,
                               unsigned* hcol, int kind, typename KeyWord<KB>::U flip)
 {
     typedef typename KeyWord<KB>::U U;
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
     constexpr int PER_VEC = 16 / KB;
-    constexpr int C = KB <= 4 ? 32 : 16;
     auto count1 = [&](U key) {
         const U k = sort_key<U>(key, kind, flip);
 #pragma unroll
