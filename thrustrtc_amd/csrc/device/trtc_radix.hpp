@@ -286,13 +286,20 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
     if (threadIdx.x < 64) {
         unsigned constant_mask = 0, rank_lds = 0;
         const double inv_n = 1.0 / (double)a.n;
+        // every count of the lane first: one round trip instead of one per
+        // (pass, quarter) of the scan
+        unsigned long long hv[KB * 4];
+#pragma unroll
+        for (int i = 0; i < KB * 4; i++) hv[i] = ld_relaxed(a.hist + i * 64 + (int)threadIdx.x);
+#pragma unroll
         for (int p = 0; p < KB; p++) {
             unsigned long long run = 0;
             bool whole = false;
             double sq = 0.0;
+#pragma unroll
             for (int c = 0; c < 4; c++) {
                 int d = c * 64 + (int)threadIdx.x;
-                unsigned long long v = ld_relaxed(a.hist + p * 256 + d);
+                unsigned long long v = hv[p * 4 + c];
                 whole = whole || v == (unsigned long long)a.n;
                 const double f = (double)v * inv_n;
                 sq += f * f;
@@ -1263,12 +1270,17 @@ TRTC_DEV void seg_radix_hist(const SegRadixArgs& a)
     if (threadIdx.x < 64) {
         unsigned rank_lds = 0;
         const double inv_n = 1.0 / (double)d.n;
+        unsigned hv[KB * 4];  // the lane's counts, loaded together
+#pragma unroll
+        for (int i = 0; i < KB * 4; i++) hv[i] = ld_relaxed(hist + i * 64 + (int)threadIdx.x);
+#pragma unroll
         for (int p = 0; p < KB; p++) {
             unsigned run = 0;
             double sq = 0.0;
+#pragma unroll
             for (int q = 0; q < 4; q++) {
                 int dd = q * 64 + (int)threadIdx.x;
-                unsigned x = ld_relaxed(hist + p * 256 + dd);
+                unsigned x = hv[p * 4 + q];
                 const double f = (double)x * inv_n;
                 sq += f * f;
                 Plus plus;
