@@ -141,8 +141,6 @@ struct SegSortDesc
     unsigned multi;                         // slot among multi-tile segments, 0xFFFFFFFF if single-tile
     unsigned long long status_row;          // multi-tile: lookback row of tile 0 (rows of 256 words)
     unsigned hist_chunks;                   // multi-tile: histogram workgroups of the segment
-    unsigned lead;                          // multi-tile: (keys % 128) / key size; tile 0 holds tile - lead
-                                            // keys so that later tiles start on 128-byte lines
 };
 
 struct SegRadixArgs

@@ -436,9 +436,13 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
     const size_t n = job.n;
     // tile 0 holds TILE - lead keys, so that with lead = the sequence's
     // offset from a 128-byte line in keys every later tile starts on a line
-    // (a 256-byte load instruction then covers two lines, not three)
-    const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
-    const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
+    // (a 256-byte load instruction then covers two lines, not three); a tile
+    // past the end (the tile count allows for the largest lead of any pass)
+    // holds no keys and publishes zero counts
+    const size_t tb0 = tile ? tile * (size_t)TILE - job.lead : 0;
+    const size_t te0 = (tile + 1) * (size_t)TILE - job.lead;
+    const size_t tile_base = tb0 < n ? tb0 : n;
+    const size_t tile_end = te0 < n ? te0 : n;
     const unsigned valid = (unsigned)(tile_end - tile_base);
     const bool full = valid == (unsigned)TILE;
     const U flip = descending ? (U)~(U)0 : (U)0;
@@ -462,9 +466,14 @@ TRTC_DEV void radix_pass_core(unsigned* tile_counter, const MakeJob& make_job, i
         for (int k = 0; k < ITEMS; k++) {
             unsigned p = (unsigned)(wpos0 + k * 64) + lane;
             if (CLAMP_TAIL) {
-                const unsigned q = p < valid ? p : valid - 1;
-                key[k] = kin[q];
-                if (VB) val[k] = vin[q];
+                if (valid) {
+                    const unsigned q = p < valid ? p : valid - 1;
+                    key[k] = kin[q];
+                    if (VB) val[k] = vin[q];
+                } else {
+                    key[k] = 0;
+                    if (VB) val[k] = 0;
+                }
             } else if (p < valid) {
                 key[k] = kin[p];
                 if (VB) val[k] = vin[p];
@@ -790,9 +799,13 @@ TRTC_DEV void radix_pass_long_core(unsigned* tile_counter, size_t num_tiles, con
         const size_t tile = job.tile;
         const size_t n = job.n;
         // tile 0 holds TILE - lead keys, so that with lead = the sequence's
-        // offset from a 128-byte line in keys every later tile starts on a line
-        const size_t tile_base = tile ? tile * (size_t)TILE - job.lead : 0;
-        const size_t tile_end = (tile + 1) * (size_t)TILE - job.lead < n ? (tile + 1) * (size_t)TILE - job.lead : n;
+        // offset from a 128-byte line in keys every later tile starts on a
+        // line; a tile past the end (room for the largest lead of any pass)
+        // holds no keys and publishes zero counts
+        const size_t tb0 = tile ? tile * (size_t)TILE - job.lead : 0;
+        const size_t te0 = (tile + 1) * (size_t)TILE - job.lead;
+        const size_t tile_base = tb0 < n ? tb0 : n;
+        const size_t tile_end = te0 < n ? te0 : n;
         const unsigned valid = (unsigned)(tile_end - tile_base);
         const bool full = valid == (unsigned)TILE;
         const unsigned wave = tid / 64;
@@ -1169,7 +1182,8 @@ TRTC_DEV void seg_radix_pass(const SegRadixArgs& a)
         j.vout = (V*)(odd ? d.vals : d.tmp_vals);
         j.n = d.n;
         j.tile = t - d.first_tile;
-        j.lead = d.lead;
+        // this pass's input's offset from a 128-byte line, in keys
+        j.lead = d.n > (unsigned long long)TILE ? (unsigned)(((size_t)j.kin & 127) / KB) : 0u;
         j.dstart64 = nullptr;
         if (d.n > (unsigned long long)TILE) {
             j.status = a.status + d.status_row * 256;
@@ -1203,7 +1217,8 @@ TRTC_DEV void seg_radix_pass_long(const SegRadixArgs& a)
         j.vout = (unsigned*)(odd ? d.vals : d.tmp_vals);
         j.n = d.n;
         j.tile = t - d.first_tile;
-        j.lead = d.lead;
+        // this pass's input's offset from a 128-byte line, in keys
+        j.lead = d.n > (unsigned long long)TILE ? (unsigned)(((size_t)j.kin & 127) / 4) : 0u;
         j.dstart64 = nullptr;
         if (d.n > (unsigned long long)TILE) {
             j.status = a.status + d.status_row * 256;

@@ -94,21 +94,26 @@ SortSchedule served_sort_schedule(int key_bytes, int val_bytes)
     return {{4096, 256, 16}, {1u << 18, 512, 16}, {kNoLimit, 1024, 16}};
 }
 
-// Keys that tile 0 of a multi-tile segment gives up (the segment's offset
-// from a 128-byte line in keys), so that its later tiles start on line
-// boundaries: a 256-byte load instruction of the pass then covers two lines,
-// not three.  Per 1024x16 pass over 128 segments of u32/u32 pairs 2^20 - 4
-// keys apart (16-byte but not line aligned): 0.648 ms without the lead,
-// 0.596 ms with it (0.587 ms for segments 2^20 keys apart).
+// The segment's offset from a 128-byte line in keys.  In every pass, tile 0
+// of a multi-tile segment gives up the lead of that pass's input buffer
+// (computed by the pass from the pointer), so that its later tiles start on
+// line boundaries: a 256-byte load instruction of the pass then covers two
+// lines, not three.  Per 1024x16 pass over 128 segments of u32/u32 pairs
+// 2^20 - 4 keys apart (16-byte but not line aligned): 0.648 ms without the
+// lead, 0.596 ms with it (0.587 ms for segments 2^20 keys apart).  Pass 0
+// reads the request's source, which a server's requests place anywhere.
 constexpr size_t kLineBytes = 128;
 unsigned seg_lead(const SortSeg& g, int KB)
 {
     return (unsigned)(((uintptr_t)g.keys & (kLineBytes - 1)) / (unsigned)KB);
 }
 
+// Tiles of a segment: a multi-tile segment has room for the largest lead of
+// any pass (a pass with a smaller one may leave its last tile empty).
 size_t seg_tiles(const SortSeg& g, size_t tile, int KB)
 {
-    return g.n > tile ? ceil_div(g.n + seg_lead(g, KB), tile) : ceil_div(g.n, tile);
+    const size_t max_lead = kLineBytes / (size_t)KB - 1;
+    return g.n > tile ? ceil_div(g.n + max_lead, tile) : ceil_div(g.n, tile);
 }
 
 bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool descending,
@@ -183,7 +188,8 @@ bool plan_sort(const std::vector<SortSeg>& segs, int KB, int VB, int kind, bool 
                 C.multi_segs++;
                 C.hist_chunks += (unsigned)ceil_div(C.segs[i].n, plan.hist_chunk);
                 // the ping-pong copy is as far past a line boundary as the
-                // segment (the buffers themselves are 256-byte aligned)
+                // segment's own buffers (the ping-pong buffers themselves are
+                // 256-byte aligned), so passes 1-3 give up the same lead
                 const size_t per = kLineBytes / (size_t)KB;
                 const size_t at = plan.tmp_elems;
                 C.tmp_off[i] = at + (seg_lead(C.segs[i], KB) + per - at % per) % per;
@@ -250,14 +256,12 @@ void fill_sort_tables(const SortPlan& plan, char* scratch, char* tmp_keys, char*
                     HC[2 * hc + 1] = c;
                     hc++;
                 }
-                d.lead = seg_lead(g, plan.KB);
             } else {
                 d.tmp_keys = d.keys;
                 d.tmp_vals = d.vals;
                 d.multi = 0xFFFFFFFFu;
                 d.status_row = 0;
                 d.hist_chunks = 0;
-                d.lead = 0;
             }
             for (size_t k = 0; k < nt; k++) TS[t++] = (unsigned)i;
         }
