@@ -195,6 +195,24 @@ def test_server_per_call_constant_digits(trtc, orc, wl, n):
         assert np.array_equal(dk.to_host(), keys), name
 
 
+def test_server_batched_source_offsets_and_empty_tiles(trtc, orc, arenas):
+    """Batched sorts whose sources start at every offset from a 128-byte line
+    (pass 0 gives up that offset in tile 0, later passes the output's), with
+    sizes that leave the last tile of some passes empty (tile counts allow for
+    the largest offset): 16374 / 40957 keys on 8192-key tiles, 294907 /
+    294949 keys on 32768-key tiles, and a one-tile 4096."""
+    say("server: batched sorts, source line offsets 0..31, empty last tiles")
+    sizes = [16374, 16385, 40957, 294907, 294949, 4096]
+    reqs = []
+    for off in range(32):
+        reqs.append((0, sizes[off % len(sizes)], 1000 + off))
+    srv = trtc.Server(64 << 30, 3, True)
+    outs = run_all(trtc, srv, arenas, reqs)
+    for r, o in zip(reqs, outs):
+        check_oracle(orc, arenas, r, o)
+    assert srv.stats()["failed"] == 0
+
+
 def test_server_small_budget_admission(trtc, orc, arenas):
     say("server: 32 MiB budget, 80 requests")
     rng = np.random.default_rng(4)
