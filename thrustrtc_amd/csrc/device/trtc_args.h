@@ -84,6 +84,7 @@ struct RadixHistArgs
     void* keys_tmp;                   // ping-pong buffers
     void* vals_tmp;
     int val_bytes;
+    int self_clear;                   // the last workgroup zeroes hist and done again after use
 };
 
 struct RadixPassArgs

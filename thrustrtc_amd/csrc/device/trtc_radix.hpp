@@ -291,6 +291,13 @@ TRTC_DEV void radix_hist(const RadixHistArgs& a)
         unsigned long long hv[KB * 4];
 #pragma unroll
         for (int i = 0; i < KB * 4; i++) hv[i] = ld_relaxed(a.hist + i * 64 + (int)threadIdx.x);
+        if (a.self_clear) {
+            // every other workgroup has added its counts and its completion:
+            // the block is left zero for the stream's next sort
+#pragma unroll
+            for (int i = 0; i < KB * 4; i++) st_relaxed(a.hist + i * 64 + (int)threadIdx.x, 0ull);
+            if (threadIdx.x == 0) st_relaxed(a.done, 0u);
+        }
 #pragma unroll
         for (int p = 0; p < KB; p++) {
             unsigned long long run = 0;

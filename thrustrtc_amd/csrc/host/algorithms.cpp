@@ -185,7 +185,19 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     if (!ktmp.p || (val_bytes && !vtmp.p) || !scratch.p) return false;
     char* sc = (char*)scratch.p;
 
-    if (!fill_u32(sc, off_cnt / 4, 0, s)) return false;  // hist + done
+    // The pre-pass accumulates into the stream's histogram block, which is
+    // zero between sorts and cleared again by the pre-pass itself, so no
+    // clearing launch precedes it (2^20 u32 sort: 6 -> 5 launches).  Under
+    // graph capture the sort keeps its own counters in the scratch and clears
+    // them first, so that the graph holds no state shared with other sorts.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = hipStreamCaptureStatusNone;
+    }
+    const bool in_capture = capturing() || cap != hipStreamCaptureStatusNone;
+    char* hb = in_capture ? nullptr : (char*)radix_hist_block(s);
+    if (!hb && !fill_u32(sc, off_cnt / 4, 0, s)) return false;  // hist + done
 
     KernelRef hk;
     std::string hname = "trtc_radix_hist_k<" + std::to_string(KB) + ">";
@@ -196,9 +208,10 @@ bool radix_sort(void* keys, void* vals, size_t n, int key_bytes, int kind, bool 
     ha.kind = kind;
     ha.descending = descending ? 1 : 0;
     ha.passes = KB;
-    ha.hist = (unsigned long long*)(sc + off_hist);
+    ha.hist = (unsigned long long*)(hb ? hb : sc + off_hist);
     ha.digit_start = (unsigned long long*)(sc + off_dstart);
-    ha.done = (unsigned*)(sc + off_done);
+    ha.done = (unsigned*)(hb ? hb + 8 * 256 * 8 : sc + off_done);
+    ha.self_clear = hb ? 1 : 0;
     ha.zero_ptr = (unsigned*)(sc + off_cnt);
     ha.zero_words = (off_dstart - off_cnt) / 4;
     // passes over constant digits (one bin holds all n keys) are skipped

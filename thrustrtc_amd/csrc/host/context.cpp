@@ -323,6 +323,32 @@ void pool_free(void* p, hipStream_t s)
     P.free_blocks[{s, it->second}].push_back(p);
 }
 
+// Per-stream histogram block of the radix sort: KB*256 64-bit counters and
+// the pre-pass's completion counter, all zero between sorts (the pre-pass's
+// last workgroup clears what it used), so a sort needs no clearing launch
+// before its pre-pass.  Sorts on one stream run one after another; another
+// stream has its own block.  Allocated and cleared on a stream's first sort,
+// kept until the process ends.
+void* radix_hist_block(hipStream_t s)
+{
+    static std::mutex mtx;
+    static std::map<hipStream_t, void*>* blocks = new std::map<hipStream_t, void*>();
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = blocks->find(s);
+    if (it != blocks->end()) return it->second;
+    void* p = nullptr;
+    if (hipMalloc(&p, kRadixHistBlockBytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (!fill_u32(p, kRadixHistBlockBytes / 4, 0, s)) {
+        hipFree(p);
+        return nullptr;
+    }
+    (*blocks)[s] = p;
+    return p;
+}
+
 // ================================================================ kernels
 namespace {
 struct Registry

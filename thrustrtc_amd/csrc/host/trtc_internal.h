@@ -164,6 +164,10 @@ void pinned_release(void* p, hipStream_t s);
 
 // Built-in utility kernels.
 bool fill_u32(void* p, size_t words, unsigned v, hipStream_t s);
+// The stream's radix histogram block (8 * 256 64-bit counters, then the
+// completion counter; zero between sorts), or null if it cannot be allocated.
+constexpr size_t kRadixHistBlockBytes = 8 * 256 * 8 + 256;
+void* radix_hist_block(hipStream_t s);
 // Zeroes words with agent-scope atomic stores (lookback state in static
 // device memory, read by the next kernel's agent-scope atomics).
 bool clear_u32_agent(void* p, size_t words, hipStream_t s);
