@@ -327,14 +327,21 @@ void pool_free(void* p, hipStream_t s)
 // the pre-pass's completion counter, all zero between sorts (the pre-pass's
 // last workgroup clears what it used), so a sort needs no clearing launch
 // before its pre-pass.  Sorts on one stream run one after another; another
-// stream has its own block.  Allocated and cleared on a stream's first sort,
-// kept until the process ends.
+// stream has its own block.  Keyed by (current device, stream): the null
+// stream is one handle on every device.  Allocated and cleared on a stream's
+// first sort, kept until the process ends.
 void* radix_hist_block(hipStream_t s)
 {
     static std::mutex mtx;
-    static std::map<hipStream_t, void*>* blocks = new std::map<hipStream_t, void*>();
+    static std::map<std::pair<int, hipStream_t>, void*>* blocks = new std::map<std::pair<int, hipStream_t>, void*>();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
     std::lock_guard<std::mutex> lk(mtx);
-    auto it = blocks->find(s);
+    const std::pair<int, hipStream_t> key(dev, s);
+    auto it = blocks->find(key);
     if (it != blocks->end()) return it->second;
     void* p = nullptr;
     if (hipMalloc(&p, kRadixHistBlockBytes) != hipSuccess) {
@@ -345,7 +352,7 @@ void* radix_hist_block(hipStream_t s)
         hipFree(p);
         return nullptr;
     }
-    (*blocks)[s] = p;
+    (*blocks)[key] = p;
     return p;
 }
 
